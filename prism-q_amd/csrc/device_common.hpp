@@ -1,0 +1,94 @@
+// device_common.hpp -- device helpers shared by the MD kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace prism {
+
+__device__ inline float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Orders this wavefront's LDS accesses (cross-lane LDS communication inside
+// one wave without a workgroup barrier).
+__device__ inline void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// float -> int64 fixed point, saturated so a pathological contribution cannot
+// wrap the accumulator.
+__device__ inline long long to_fixed(float v, int bits) {
+    double d = (double)v * (double)(1ll << bits);
+    d = fmin(fmax(d, -4.0e18), 4.0e18);
+    return (long long)rint(d);
+}
+__device__ inline void add_fixed(long long *acc, float v, int bits) {
+    atomicAdd((unsigned long long *)acc, (unsigned long long)to_fixed(v, bits));
+}
+__device__ inline float from_fixed(long long v, int bits) {
+    return (float)((double)v * (1.0 / (double)(1ll << bits)));
+}
+
+// Philox4x32-10 (Salmon et al., SC'11), Random123 constants.
+__device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+    }
+}
+
+// Three standard normals of the Langevin stream: key (seed low word, stream),
+// counter (atom, step low word, step bits 32..47 << 16 | purpose << 8, 0);
+// Box-Muller with u1 = (r0 + 1) 2^-32, u2 = r1 2^-32 (and r2, r3).
+__device__ inline void langevin_normals(uint32_t seed_lo, uint32_t stream, uint32_t atom, unsigned long long step,
+                                        uint32_t purpose, float out[3]) {
+    uint32_t c[4] = {atom, (uint32_t)step, (uint32_t)(((step >> 32) & 0xFFFFull) << 16) | ((purpose & 0xFFu) << 8), 0u};
+    philox4x32_10(c, seed_lo, stream);
+    const float s = 2.3283064365386963e-10f;
+    const float a = sqrtf(-2.0f * logf(fmaf((float)c[0], s, s)));
+    const float b = sqrtf(-2.0f * logf(fmaf((float)c[2], s, s)));
+    float sa, ca, sb, cb;
+    sincosf(6.283185307179586f * ((float)c[1] * s), &sa, &ca);
+    sincosf(6.283185307179586f * ((float)c[3] * s), &sb, &cb);
+    out[0] = a * ca;
+    out[1] = a * sa;
+    out[2] = b * cb;
+}
+
+// Director schedule (protocol_director.cu:58-123): temperature table and the
+// equilibration friction ramp at schedule step `step`.
+__device__ inline void schedule(const SysDesc &D, unsigned long long step, float &T, float &gamma) {
+    const long long s = (long long)step;
+    if (s < D.cold_hold_end) T = D.t_start;
+    else if (s < D.ramp_end) T = D.t_start + (float)(s - D.cold_hold_end) / (float)(D.ramp_end - D.cold_hold_end) * (D.t_end - D.t_start);
+    else if (s < D.warm_hold_end) T = D.t_end;
+    else if (s < D.ramp_down_end) T = D.t_end - (float)(s - D.warm_hold_end) / (float)(D.ramp_down_end - D.warm_hold_end) * (D.t_end - D.t_start);
+    else T = D.warm_tail ? D.t_end : D.t_start;
+    gamma = D.gamma_base;
+    if (s < D.equilibration_steps) {
+        const float decay = expf(-3.0f * (float)s / (float)D.equilibration_steps);
+        gamma = D.gamma_eq * decay + D.gamma_base * (1.0f - decay);
+    }
+}
+
+}  // namespace prism

@@ -1,0 +1,878 @@
+// engine.hip -- host engine and C ABI of the MI355X classical MD step.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <utility>
+
+namespace prism {
+
+namespace {
+
+int pad64(int n) { return (n + BLOCK - 1) / BLOCK * BLOCK; }
+
+// Host Maxwell-Boltzmann draw from the Langevin Philox stream (purpose 3),
+// COM velocity removed (role of generate_maxwell_boltzmann_velocities,
+// fused_engine.rs:16066-16116; the reference's StdRng stream is not reproduced).
+void host_philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = (uint32_t)p1;
+        c[2] = n2;
+        c[3] = (uint32_t)p0;
+    }
+}
+
+void maxwell_boltzmann(const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p, std::vector<float> &v) {
+    const int n = t->n_atoms;
+    v.assign(3 * (size_t)n, 0.0f);
+    std::vector<double> vd(3 * (size_t)n);
+    double mom[3] = {0, 0, 0}, mt = 0.0;
+    for (int i = 0; i < n; ++i) {
+        uint32_t c[4] = {(uint32_t)i, 0u, 3u << 8, 0u};
+        host_philox(c, (uint32_t)p.seed, p.stream_id);
+        const double s = 2.3283064365386963e-10;
+        const double a = std::sqrt(-2.0 * std::log(((double)c[0] + 1.0) * s)), b = std::sqrt(-2.0 * std::log(((double)c[2] + 1.0) * s));
+        const double z[3] = {a * std::cos(6.283185307179586 * c[1] * s), a * std::sin(6.283185307179586 * c[1] * s),
+                             b * std::cos(6.283185307179586 * c[3] * s)};
+        const double m = t->masses[i];
+        const double sig = std::sqrt(KCAL_TO_AMU_A2_PS2 * KB * p.temperature_K / m);
+        for (int k = 0; k < 3; ++k) {
+            vd[3 * i + k] = sig * z[k];
+            mom[k] += m * vd[3 * i + k];
+        }
+        mt += m;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) v[3 * i + k] = (float)(vd[3 * i + k] - mom[k] / mt);
+}
+
+}  // namespace
+
+template <class T> T *Engine::dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) {
+        alloc_failed_ = true;
+        return nullptr;
+    }
+    allocs_.push_back(p);
+    hipMemsetAsync(p, 0, n * sizeof(T), stream_);
+    return (T *)p;
+}
+
+template <class T> int Engine::upload(T *d, const std::vector<T> &h) {
+    if (h.empty()) return PRISM_MD_OK;
+    return hipcheck(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream_), "upload");
+}
+
+int Engine::fail(int rc, const std::string &msg) {
+    err_ = msg;
+    return rc;
+}
+
+int Engine::hipcheck(hipError_t e, const char *what) {
+    if (e == hipSuccess) return PRISM_MD_OK;
+    return fail(PRISM_MD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int Engine::d2h(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return PRISM_MD_OK;
+    if (int rc = hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_), "device-to-host copy")) return rc;
+    return hipcheck(hipStreamSynchronize(stream_), "device-to-host sync");
+}
+
+Engine::~Engine() {
+    if (stream_) hipStreamSynchronize(stream_);
+    if (graph_) hipGraphExecDestroy(graph_);
+    for (void *p : allocs_) hipFree(p);
+    if (own_stream_ && stream_) hipStreamDestroy(stream_);
+}
+
+static int validate(const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p, std::string &why) {
+    const int n = t->n_atoms;
+    auto bad = [&](const char *m) { why = m; return PRISM_MD_E_INVALID; };
+    if (n <= 0 || !t->positions || !t->masses || !t->charges || !t->sigma || !t->epsilon) return bad("atom arrays");
+    if (!t->exclusion_offsets || t->exclusion_offsets[0] != 0) return bad("exclusion offsets");
+    for (int i = 0; i < n; ++i) {
+        if (!(t->masses[i] > 0.0f) || !std::isfinite(t->positions[3 * i]) || !std::isfinite(t->positions[3 * i + 1]) ||
+            !std::isfinite(t->positions[3 * i + 2]))
+            return bad("mass or position");
+        if (t->exclusion_offsets[i + 1] < t->exclusion_offsets[i]) return bad("exclusion offsets not monotone");
+    }
+    const int ne = t->exclusion_offsets[n];
+    if (ne > 0 && !t->exclusion_atoms) return bad("exclusion atoms");
+    for (int e = 0; e < ne; ++e)
+        if (t->exclusion_atoms[e] < 0 || t->exclusion_atoms[e] >= n) return bad("exclusion index");
+    auto idx_ok = [&](const int32_t *a, int count, int arity) {
+        for (int k = 0; k < count * arity; ++k)
+            if (a[k] < 0 || a[k] >= n) return false;
+        return true;
+    };
+    if (t->n_bonds > 0 && (!t->bond_atoms || !t->bond_r0 || !t->bond_k || !idx_ok(t->bond_atoms, t->n_bonds, 2))) return bad("bonds");
+    if (t->n_angles > 0 && (!t->angle_atoms || !t->angle_theta0 || !t->angle_k || !idx_ok(t->angle_atoms, t->n_angles, 3)))
+        return bad("angles");
+    if (t->n_dihedrals > 0 && (!t->dihedral_atoms || !t->dihedral_periodicity || !t->dihedral_phase || !t->dihedral_k ||
+                               !idx_ok(t->dihedral_atoms, t->n_dihedrals, 4)))
+        return bad("dihedrals");
+    if (t->n_exceptions > 0 && (!t->exception_atoms || !t->exception_charge_product || !t->exception_sigma ||
+                                !t->exception_epsilon || !idx_ok(t->exception_atoms, t->n_exceptions, 2)))
+        return bad("exceptions");
+    std::vector<char> owned(n, 0);
+    for (int c = 0; c < t->n_clusters; ++c) {
+        const int nh = t->cluster_n_hydrogens[c], a = t->cluster_central[c];
+        if (nh < 1 || nh > 3 || a < 0 || a >= n || owned[a]) return bad("cluster central");
+        owned[a] = 1;
+        if (!(t->cluster_inv_mass_central[c] > 0.0f) || !(t->cluster_inv_mass_h[c] > 0.0f)) return bad("cluster masses");
+        for (int h = 0; h < nh; ++h) {
+            const int b = t->cluster_hydrogens[3 * c + h];
+            if (b < 0 || b >= n || owned[b] || !(t->cluster_bond_lengths[3 * c + h] > 0.0f)) return bad("cluster hydrogen");
+            owned[b] = 1;
+        }
+    }
+    if (!(p.dt_ps > 0.0) || !(p.cutoff_A > 0.0) || !(p.skin_A >= 0.0) || p.rebuild_every < 1 || !(p.temperature_K >= 0.0) ||
+        !(p.gamma_ps >= 0.0) || p.constraint_max_iter < 1 || !(p.shake_tol > 0.0) || !(p.rattle_tol > 0.0))
+        return bad("parameters");
+    return PRISM_MD_OK;
+}
+
+int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(PRISM_MD_E_NODEVICE, "no HIP device");
+    if (stream) {
+        stream_ = (hipStream_t)stream;
+    } else {
+        if (int rc = hipcheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream")) return rc;
+        own_stream_ = true;
+    }
+    for (const auto &sp : systems) {
+        std::string why;
+        if (validate(sp.topo, sp.params, why) != PRISM_MD_OK) return fail(PRISM_MD_E_INVALID, "invalid topology/params: " + why);
+    }
+    // ---- arena layout
+    int A = 0, W = 0;
+    std::vector<float4> hx, hv;
+    std::vector<float> hmass, hq, hsig, heps;
+    std::vector<int> hatom_sys, hblock_sys, hwave_sys;
+    std::vector<int> skip_off{0}, skip_idx, ex_off{0}, ex_idx;
+    std::vector<Group> hgroups;
+    std::vector<GroupParam> hgp;
+    std::vector<Bond> hb;
+    std::vector<Angle> ha;
+    std::vector<Dihedral> hd;
+    std::vector<Exception14> hx14;
+    for (int s = 0; s < (int)systems.size(); ++s) {
+        const PrismMdTopologyV1 *t = systems[s].topo;
+        const PrismMdParamsV1 &p = systems[s].params;
+        const int n = t->n_atoms;
+        HostSys H{};
+        H.atom_begin = A;
+        H.n_atoms = n;
+        H.n_slots = pad64(n);
+        H.block_begin = A / BLOCK;
+        H.n_blocks = H.n_slots / BLOCK;
+        A += H.n_slots;
+        std::vector<float> v0;
+        maxwell_boltzmann(t, p, v0);
+        double mt = 0.0;
+        for (int i = 0; i < H.n_slots; ++i) {
+            if (i < n) {
+                hx.push_back(make_float4(t->positions[3 * i], t->positions[3 * i + 1], t->positions[3 * i + 2], 0.0f));
+                hv.push_back(make_float4(v0[3 * i], v0[3 * i + 1], v0[3 * i + 2], 0.0f));
+                hmass.push_back(t->masses[i]);
+                hq.push_back(t->charges[i]);
+                hsig.push_back(t->sigma[i]);
+                heps.push_back(t->epsilon[i]);
+                hatom_sys.push_back(s);
+                mt += t->masses[i];
+            } else {
+                hx.push_back(make_float4(0, 0, 0, 0));
+                hv.push_back(make_float4(0, 0, 0, 0));
+                hmass.push_back(1.0f);
+                hq.push_back(0.0f);
+                hsig.push_back(0.0f);
+                heps.push_back(0.0f);
+                hatom_sys.push_back(-1);
+            }
+        }
+        for (int b = 0; b < H.n_blocks; ++b) hblock_sys.push_back(s);
+        // exception partners per atom
+        std::vector<std::vector<int>> expart(n);
+        for (int e = 0; e < t->n_exceptions; ++e) {
+            const int i = t->exception_atoms[2 * e], j = t->exception_atoms[2 * e + 1];
+            expart[i].push_back(j);
+            expart[j].push_back(i);
+        }
+        for (int i = 0; i < H.n_slots; ++i) {
+            if (i < n) {
+                std::vector<int> sk, ex;
+                auto &xp = expart[i];
+                std::sort(xp.begin(), xp.end());
+                for (int e = t->exclusion_offsets[i]; e < t->exclusion_offsets[i + 1]; ++e) {
+                    const int j = t->exclusion_atoms[e];
+                    sk.push_back(j);
+                    if (!std::binary_search(xp.begin(), xp.end(), j)) ex.push_back(j);  // fused_engine.rs:11718-11731
+                }
+                for (int j : xp) sk.push_back(j);
+                std::sort(sk.begin(), sk.end());
+                sk.erase(std::unique(sk.begin(), sk.end()), sk.end());
+                std::sort(ex.begin(), ex.end());
+                ex.erase(std::unique(ex.begin(), ex.end()), ex.end());
+                for (int j : sk) skip_idx.push_back(H.atom_begin + j);
+                for (int j : ex) ex_idx.push_back(H.atom_begin + j);
+            }
+            skip_off.push_back((int)skip_idx.size());
+            ex_off.push_back((int)ex_idx.size());
+        }
+        // constraint groups: clusters, then lone atoms, padded to whole waves
+        std::vector<char> in_cluster(n, 0);
+        std::vector<Group> g;
+        std::vector<GroupParam> gpp;
+        for (int c = 0; c < t->n_clusters; ++c) {
+            Group G{{-1, -1, -1, -1}};
+            GroupParam Q{};
+            const int nh = t->cluster_n_hydrogens[c];
+            G.atom[0] = H.atom_begin + t->cluster_central[c];
+            in_cluster[t->cluster_central[c]] = 1;
+            for (int h = 0; h < nh; ++h) {
+                G.atom[1 + h] = H.atom_begin + t->cluster_hydrogens[3 * c + h];
+                in_cluster[t->cluster_hydrogens[3 * c + h]] = 1;
+                Q.d[h] = t->cluster_bond_lengths[3 * c + h];
+            }
+            Q.n_h = nh;
+            Q.inv_mass_c = t->cluster_inv_mass_central[c];
+            Q.inv_mass_h = t->cluster_inv_mass_h[c];
+            g.push_back(G);
+            gpp.push_back(Q);
+        }
+        for (int i = 0; i < n; ++i)
+            if (!in_cluster[i]) {
+                Group G{{H.atom_begin + i, -1, -1, -1}};
+                GroupParam Q{};
+                Q.inv_mass_c = 1.0f / t->masses[i];
+                g.push_back(G);
+                gpp.push_back(Q);
+            }
+        const int nw = ((int)g.size() + 63) / 64;
+        H.wave_begin = W;
+        H.n_waves = nw;
+        W += nw;
+        while ((int)g.size() < nw * 64) {
+            g.push_back(Group{{-1, -1, -1, -1}});
+            gpp.push_back(GroupParam{});
+        }
+        hgroups.insert(hgroups.end(), g.begin(), g.end());
+        hgp.insert(hgp.end(), gpp.begin(), gpp.end());
+        for (int w = 0; w < nw; ++w) hwave_sys.push_back(s);
+        const int o = H.atom_begin;
+        for (int b = 0; b < t->n_bonds; ++b)
+            hb.push_back(Bond{o + t->bond_atoms[2 * b], o + t->bond_atoms[2 * b + 1], t->bond_r0[b], t->bond_k[b]});
+        for (int a = 0; a < t->n_angles; ++a)
+            ha.push_back(Angle{o + t->angle_atoms[3 * a], o + t->angle_atoms[3 * a + 1], o + t->angle_atoms[3 * a + 2],
+                               t->angle_theta0[a], t->angle_k[a], {0, 0, 0}});
+        for (int d = 0; d < t->n_dihedrals; ++d)
+            hd.push_back(Dihedral{o + t->dihedral_atoms[4 * d], o + t->dihedral_atoms[4 * d + 1], o + t->dihedral_atoms[4 * d + 2],
+                                  o + t->dihedral_atoms[4 * d + 3], t->dihedral_periodicity[d], t->dihedral_phase[d],
+                                  t->dihedral_k[d], 0});
+        for (int e = 0; e < t->n_exceptions; ++e)
+            hx14.push_back(Exception14{o + t->exception_atoms[2 * e], o + t->exception_atoms[2 * e + 1],
+                                       t->exception_charge_product[e], t->exception_sigma[e], t->exception_epsilon[e], {0, 0, 0}});
+        // reference cell grid from the creation-time coordinates, float
+        // arithmetic of fused_engine.rs:12285-12330
+        H.x0.assign(t->positions, t->positions + 3 * (size_t)n);
+        float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], t->positions[3 * i + c]);
+                hi[c] = std::max(hi[c], t->positions[3 * i + c]);
+            }
+        int dims[3];
+        H.grid_ok = 1;
+        for (int c = 0; c < 3; ++c) {
+            const float extent = (hi[c] - lo[c]) + 2.0f * 10.0f;
+            const long v = std::max(1L, (long)std::ceil(extent / 10.0f));
+            if (v > 32) H.grid_ok = 0;
+            dims[c] = (int)v;
+        }
+        H.grid = RefGrid{lo[0] - 10.0f, lo[1] - 10.0f, lo[2] - 10.0f, dims[0], dims[1], dims[2]};
+        H.dof = std::max(1.0, 3.0 * n - (double)std::accumulate(t->cluster_n_hydrogens, t->cluster_n_hydrogens + t->n_clusters, 0) - 3.0);
+        SysDesc D{};
+        D.atom_begin = H.atom_begin;
+        D.n_atoms = n;
+        D.block_begin = H.block_begin;
+        D.n_blocks = H.n_blocks;
+        D.wave_begin = H.wave_begin;
+        D.n_waves = H.n_waves;
+        D.active = 1;
+        D.rebuild_every = p.rebuild_every;
+        D.com_every = p.com_remove_every;
+        D.constraint_max_iter = p.constraint_max_iter;
+        D.sort_bits = 10;
+        D.dt = (float)p.dt_ps;
+        D.cutoff = (float)p.cutoff_A;
+        D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);
+        D.list_cutoff = (float)(p.cutoff_A + p.skin_A);
+        D.half_skin2 = (float)(0.25 * p.skin_A * p.skin_A);
+        D.shake_tol = p.shake_tol;
+        D.rattle_tol = p.rattle_tol;
+        D.inv_total_mass = 1.0 / mt;
+        D.t_start = (float)p.temperature_K;
+        D.t_end = (float)p.end_temperature_K;
+        D.gamma_base = (float)p.gamma_ps;
+        D.gamma_eq = (float)p.equilibration_gamma_ps;
+        D.cold_hold_end = p.cold_hold_end;
+        D.ramp_end = p.ramp_end;
+        D.warm_hold_end = p.warm_hold_end;
+        D.ramp_down_end = p.ramp_down_end;
+        D.warm_tail = p.warm_tail;
+        D.equilibration_steps = p.equilibration_steps;
+        D.seed_lo = (uint32_t)p.seed;
+        D.stream = p.stream_id;
+        desc_.push_back(D);
+        sys_.push_back(std::move(H));
+    }
+    const int B = A / BLOCK;
+    // ---- device allocation
+    P_.n_sys = (int)systems.size();
+    P_.n_atoms_arena = A;
+    P_.n_blocks_arena = B;
+    P_.n_waves_arena = W;
+    P_.x = dalloc<float4>(A);
+    P_.v = dalloc<float4>(A);
+    P_.xs = dalloc<float4>(A);
+    P_.x_build = dalloc<float4>(A);
+    P_.f = dalloc<long long>((size_t)A * 4);
+    float *d_mass = dalloc<float>(A), *d_q = dalloc<float>(A), *d_sig = dalloc<float>(A), *d_eps = dalloc<float>(A);
+    int *d_atom_sys = dalloc<int>(A);
+    P_.perm = dalloc<int>(A);
+    P_.slot_of = dalloc<int>(A);
+    P_.xq = dalloc<float4>(A);
+    P_.lj = dalloc<float2>(A);
+    P_.bb_c = dalloc<float4>(B);
+    P_.bb_h = dalloc<float4>(B);
+    int *d_block_sys = dalloc<int>(B);
+    P_.row_n = dalloc<int>(B);
+    P_.row_j = dalloc<int>((size_t)B * MAX_ROW);
+    P_.row_m = dalloc<int>((size_t)B * MAX_ROW);
+    P_.masks = dalloc<unsigned long long>((size_t)B * MAX_MASKS * 64);
+    P_.unit_count = dalloc<int>(B);
+    P_.units = dalloc<int>((size_t)B * UNITS_PER_BLOCK);
+    int *d_skip_off = dalloc<int>(skip_off.size()), *d_skip_idx = dalloc<int>(skip_idx.size());
+    int *d_ex_off = dalloc<int>(ex_off.size()), *d_ex_idx = dalloc<int>(ex_idx.size());
+    Group *d_groups = dalloc<Group>((size_t)W * 64);
+    GroupParam *d_gp = dalloc<GroupParam>((size_t)W * 64);
+    int *d_wave_sys = dalloc<int>(W);
+    Bond *d_b = dalloc<Bond>(hb.size());
+    Angle *d_a = dalloc<Angle>(ha.size());
+    Dihedral *d_d = dalloc<Dihedral>(hd.size());
+    Exception14 *d_e = dalloc<Exception14>(hx14.size());
+    d_desc_ = dalloc<SysDesc>(desc_.size());
+    P_.sst = dalloc<SysState>(desc_.size());
+    P_.keys_a = dalloc<unsigned int>(A);
+    P_.keys_b = dalloc<unsigned int>(A);
+    P_.vals_a = dalloc<int>(A);
+    P_.vals_b = dalloc<int>(A);
+    P_.com_partial = dalloc<double>((size_t)W * 3);
+    P_.counters = dalloc<EngineCounters>(1);
+    d_count_ = dalloc<unsigned long long>(1);
+    d_red_ = dalloc<double>(desc_.size() * 4);
+    d_noise_ = dalloc<float>((size_t)2 * A * 3);
+    d_vmin_ = dalloc<float4>(A);
+    d_fire_ = dalloc<double>((size_t)(A / 256 + 1) * 4);
+    if (alloc_failed_) return fail(PRISM_MD_E_NOMEM, "device allocation failed");
+    P_.mass = d_mass; P_.charge = d_q; P_.sigma = d_sig; P_.eps = d_eps; P_.atom_sys = d_atom_sys;
+    P_.block_sys = d_block_sys;
+    P_.skip_off = d_skip_off; P_.skip_idx = d_skip_idx; P_.ex_off = d_ex_off; P_.ex_idx = d_ex_idx;
+    P_.groups = d_groups; P_.gparams = d_gp; P_.wave_sys = d_wave_sys;
+    P_.bonds = d_b; P_.angles = d_a; P_.dih = d_d; P_.exc = d_e;
+    P_.n_bonds = (int)hb.size(); P_.n_angles = (int)ha.size(); P_.n_dih = (int)hd.size(); P_.n_exc = (int)hx14.size();
+    P_.sys = d_desc_;
+    int rc = PRISM_MD_OK;
+    if ((rc = upload(P_.x, hx)) || (rc = upload(P_.v, hv)) || (rc = upload(d_mass, hmass)) || (rc = upload(d_q, hq)) ||
+        (rc = upload(d_sig, hsig)) || (rc = upload(d_eps, heps)) || (rc = upload(d_atom_sys, hatom_sys)) ||
+        (rc = upload(d_block_sys, hblock_sys)) || (rc = upload(d_skip_off, skip_off)) || (rc = upload(d_skip_idx, skip_idx)) ||
+        (rc = upload(d_ex_off, ex_off)) || (rc = upload(d_ex_idx, ex_idx)) || (rc = upload(d_groups, hgroups)) ||
+        (rc = upload(d_gp, hgp)) || (rc = upload(d_wave_sys, hwave_sys)) || (rc = upload(d_b, hb)) || (rc = upload(d_a, ha)) ||
+        (rc = upload(d_d, hd)) || (rc = upload(d_e, hx14)) || (rc = upload(d_desc_, desc_)))
+        return rc;
+    std::vector<SysState> st(desc_.size());
+    for (auto &S : st) {
+        std::memset(&S, 0, sizeof(S));
+        S.rebuild_step = -1;
+    }
+    if ((rc = upload(P_.sst, st))) return rc;
+    launch_kinetic(P_, d_red_, stream_);  // initial COM velocity (zero by construction)
+    if ((rc = refresh_all())) return rc;
+    return check_status();
+}
+
+// Rebuild the lists at the current positions and recompute the carried force.
+int Engine::refresh_all() {
+    for (int s = 0; s < (int)sys_.size(); ++s) {
+        const int one = 1;
+        if (int rc = hipcheck(hipMemcpyAsync((char *)(P_.sst + s) + offsetof(SysState, rebuild_flag), &one, sizeof(int),
+                                             hipMemcpyHostToDevice, stream_),
+                              "flag"))
+            return rc;
+    }
+    launch_sort(P_, -1, stream_);
+    launch_rows(P_, stream_);
+    full_force();
+    return hipcheck(hipGetLastError(), "refresh");
+}
+
+void Engine::full_force() {
+    launch_zero_forces(P_, stream_);
+    launch_nonbonded(P_, false, stream_);
+    launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
+    ++force_evals_;
+}
+
+void Engine::launch_step(const float *noise_n, const float *noise_np1) {
+    launch_first_half(P_, noise_n, stream_);
+    launch_sort(P_, 0, stream_);
+    launch_rows(P_, stream_);
+    launch_nonbonded(P_, false, stream_);
+    launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
+    launch_second_half(P_, noise_np1, stream_);
+    ++force_evals_;
+}
+
+int Engine::check_status() {
+    if (int rc = hipcheck(hipStreamSynchronize(stream_), "synchronize")) return rc;
+    std::vector<SysState> st(sys_.size());
+    if (int rc = d2h(st.data(), P_.sst, st.size() * sizeof(SysState))) return rc;
+    for (size_t s = 0; s < st.size(); ++s) {
+        const int b = st[s].status;
+        if (!b) continue;
+        const std::string where = " (system " + std::to_string(s) + ", step " + std::to_string(st[s].step) + ")";
+        if (b & ST_CONSTRAINT) return fail(PRISM_MD_E_CONSTRAINT, "SHAKE/RATTLE did not converge or state non-finite" + where);
+        if (b & ST_NONFINITE) return fail(PRISM_MD_E_NONFINITE, "non-finite state" + where);
+        if (b & ST_EXCEPTION_COINCIDENT) return fail(PRISM_MD_E_NONFINITE, "coincident exception atoms" + where);
+        if (b & ST_ROW_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "tile row overflow" + where);
+        if (b & ST_MASK_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "exclusion-mask overflow" + where);
+        return fail(PRISM_MD_E_INVALID, "device status " + std::to_string(b) + where);
+    }
+    return PRISM_MD_OK;
+}
+
+int Engine::set_state(int s, const float *x, const float *v, int flags) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    const HostSys &H = sys_[s];
+    const int n = H.n_atoms;
+    std::vector<float> hx, hv;
+    auto fetch = [&](const float *src, std::vector<float> &dst) -> int {
+        dst.resize(3 * (size_t)n);
+        if (flags == PRISM_MD_DEVICE_PTR)
+            return hipcheck(hipMemcpy(dst.data(), src, dst.size() * sizeof(float), hipMemcpyDeviceToHost), "state fetch");
+        std::memcpy(dst.data(), src, dst.size() * sizeof(float));
+        return PRISM_MD_OK;
+    };
+    std::vector<float4> pk(n);
+    if (x) {
+        if (int rc = fetch(x, hx)) return rc;
+        for (int i = 0; i < n; ++i) {
+            if (!std::isfinite(hx[3 * i]) || !std::isfinite(hx[3 * i + 1]) || !std::isfinite(hx[3 * i + 2]))
+                return fail(PRISM_MD_E_NONFINITE, "non-finite position");
+            pk[i] = make_float4(hx[3 * i], hx[3 * i + 1], hx[3 * i + 2], 0.0f);
+        }
+        if (int rc = hipcheck(hipMemcpyAsync(P_.x + H.atom_begin, pk.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream_),
+                              "set x"))
+            return rc;
+    }
+    if (v) {
+        if (int rc = fetch(v, hv)) return rc;
+        for (int i = 0; i < n; ++i) pk[i] = make_float4(hv[3 * i], hv[3 * i + 1], hv[3 * i + 2], 0.0f);
+        if (int rc = hipcheck(hipMemcpyAsync(P_.v + H.atom_begin, pk.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream_),
+                              "set v"))
+            return rc;
+        launch_kinetic(P_, d_red_, stream_);
+    }
+    if (x)
+        if (int rc = refresh_all()) return rc;
+    return check_status();
+}
+
+int Engine::get_state(int s, float *x, float *v, int flags) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    const HostSys &H = sys_[s];
+    const int n = H.n_atoms;
+    if (int rc = hipcheck(hipStreamSynchronize(stream_), "sync")) return rc;
+    std::vector<float4> pk(n);
+    std::vector<float> flat(3 * (size_t)n);
+    for (int which = 0; which < 2; ++which) {
+        float *dst = which ? v : x;
+        if (!dst) continue;
+        if (int rc = d2h(pk.data(), (which ? P_.v : P_.x) + H.atom_begin, n * sizeof(float4))) return rc;
+        for (int i = 0; i < n; ++i) {
+            flat[3 * i] = pk[i].x;
+            flat[3 * i + 1] = pk[i].y;
+            flat[3 * i + 2] = pk[i].z;
+        }
+        if (flags == PRISM_MD_DEVICE_PTR) {
+            if (int rc = hipcheck(hipMemcpyAsync(dst, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice, stream_),
+                                  "put state"))
+                return rc;
+            if (int rc = hipcheck(hipStreamSynchronize(stream_), "put state")) return rc;
+        } else {
+            std::memcpy(dst, flat.data(), flat.size() * sizeof(float));
+        }
+    }
+    return PRISM_MD_OK;
+}
+
+int Engine::compute_forces(unsigned mask, int s, float *F_out, int64_t *F_fixed, double *E_terms) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    mask &= PRISM_MD_TERM_ALL;
+    for (int q = 0; q < (int)sys_.size(); ++q)
+        if (int rc = hipcheck(hipMemsetAsync(&P_.sst[q].energy[0], 0, sizeof(long long) * 8, stream_), "energy zero")) return rc;
+    launch_zero_forces(P_, stream_);
+    if (mask & PRISM_MD_TERM_NONBONDED) launch_nonbonded(P_, E_terms != nullptr, stream_);
+    launch_bonded(P_, mask, E_terms != nullptr, stream_);
+    if (int rc = hipcheck(hipGetLastError(), "force launch")) return rc;
+    if (int rc = check_status()) return rc;
+    const HostSys &H = sys_[s];
+    const int n = H.n_atoms;
+    std::vector<long long> acc((size_t)n * 4);
+    if (int rc = d2h(acc.data(), P_.f + (size_t)H.atom_begin * 4, acc.size() * sizeof(long long))) return rc;
+    const double inv = 1.0 / (double)(1ll << FORCE_FIXED_BITS);
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            if (F_out) F_out[3 * i + c] = (float)((double)acc[(size_t)i * 4 + c] * inv);
+            if (F_fixed) F_fixed[3 * i + c] = acc[(size_t)i * 4 + c];
+        }
+    if (E_terms) {
+        SysState S;
+        if (int rc = d2h(&S, P_.sst + s, sizeof(S))) return rc;
+        for (int k = 0; k < PRISM_MD_N_TERMS; ++k) E_terms[k] = (double)S.energy[k] / (double)(1ll << ENERGY_FIXED_BITS);
+    }
+    if (mask != PRISM_MD_TERM_ALL) full_force();  // restore the carried F(x)
+    return check_status();
+}
+
+int Engine::step(uint64_t n_steps, const float *noise) {
+    if (noise && sys_.size() != 1) return fail(PRISM_MD_E_UNSUPPORTED, "explicit noise needs a single system");
+    const int n = noise ? sys_[0].n_atoms : 0;
+    const size_t per_half = 3 * (size_t)n;
+    for (uint64_t k = 0; k < n_steps; ++k) {
+        const float *nn = nullptr, *np = nullptr;
+        if (noise) {
+            if (int rc = hipcheck(hipMemcpyAsync(d_noise_, noise + (size_t)k * 2 * per_half, 2 * per_half * sizeof(float),
+                                                 hipMemcpyHostToDevice, stream_),
+                                  "noise upload"))
+                return rc;
+            nn = d_noise_;
+            np = d_noise_ + per_half;
+        }
+        launch_step(nn, np);
+        if (noise)  // the staging buffer is reused next step
+            if (int rc = hipcheck(hipStreamSynchronize(stream_), "noise sync")) return rc;
+    }
+    host_steps_ += n_steps;
+    if (int rc = hipcheck(hipGetLastError(), "step launch")) return rc;
+    return check_status();
+}
+
+int Engine::step_graph(uint64_t n_steps, int graph_steps) {
+    if (graph_steps < 1) return fail(PRISM_MD_E_INVALID, "graph_steps");
+    if (!graph_ || graph_steps_ != graph_steps) {
+        if (graph_) hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+        hipGraph_t g;
+        if (int rc = hipcheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture")) return rc;
+        for (int k = 0; k < graph_steps; ++k) launch_step(nullptr, nullptr);
+        if (int rc = hipcheck(hipStreamEndCapture(stream_, &g), "end capture")) return rc;
+        const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (int rc = hipcheck(e, "instantiate")) return rc;
+        graph_steps_ = graph_steps;
+    }
+    uint64_t done = 0;
+    for (; done + graph_steps <= n_steps; done += graph_steps)
+        if (int rc = hipcheck(hipGraphLaunch(graph_, stream_), "graph launch")) return rc;
+    for (; done < n_steps; ++done) launch_step(nullptr, nullptr);
+    host_steps_ += n_steps;
+    force_evals_ += n_steps;
+    if (int rc = hipcheck(hipGetLastError(), "graph step")) return rc;
+    return check_status();
+}
+
+int Engine::energies(int s, double out[3]) {
+    double E[PRISM_MD_N_TERMS];
+    if (int rc = compute_forces(PRISM_MD_TERM_ALL, s, nullptr, nullptr, E)) return rc;
+    launch_kinetic(P_, d_red_, stream_);
+    double red[4];
+    if (int rc = hipcheck(hipMemcpyAsync(red, d_red_ + 4 * s, sizeof(red), hipMemcpyDeviceToHost, stream_), "ke copy")) return rc;
+    if (int rc = hipcheck(hipStreamSynchronize(stream_), "sync")) return rc;
+    out[0] = E[0] + E[1] + E[2] + E[3] + E[4];
+    out[1] = red[0];
+    out[2] = 2.0 * red[0] / (sys_[s].dof * KB);
+    return PRISM_MD_OK;
+}
+
+int Engine::build_pairs(int s, int mode) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    if (mode != PRISM_MD_PAIRS_REFERENCE && mode != PRISM_MD_PAIRS_BRUTE) return fail(PRISM_MD_E_INVALID, "pair mode");
+    const HostSys &H = sys_[s];
+    if (mode == PRISM_MD_PAIRS_REFERENCE && !H.grid_ok)
+        return fail(PRISM_MD_E_CAPACITY, "reference cell grid needs more than 32 cells per axis");
+    if (int rc = refresh_all()) return rc;
+    if (int rc = check_status()) return rc;
+    unsigned long long count = 0;
+    hipMemsetAsync(d_count_, 0, sizeof(count), stream_);
+    launch_emit_pairs(P_, mode, H.grid, nullptr, nullptr, d_count_, 0ull, stream_);
+    if (int rc = d2h(&count, d_count_, sizeof(count))) return rc;
+    int *di = nullptr, *dj = nullptr;
+    if (int rc = hipcheck(hipMalloc(&di, std::max<size_t>(1, count) * sizeof(int)), "pair alloc")) return rc;
+    if (int rc = hipcheck(hipMalloc(&dj, std::max<size_t>(1, count) * sizeof(int)), "pair alloc")) {
+        hipFree(di);
+        return rc;
+    }
+    hipMemsetAsync(d_count_, 0, sizeof(count), stream_);
+    launch_emit_pairs(P_, mode, H.grid, di, dj, d_count_, count, stream_);
+    std::vector<int> hi(count), hj(count);
+    int rc = d2h(hi.data(), di, count * sizeof(int));
+    if (!rc) rc = d2h(hj.data(), dj, count * sizeof(int));
+    hipFree(di);
+    hipFree(dj);
+    if (rc) return rc;
+    // the reference audits per-atom list saturation (fused_engine.rs:16722-16735)
+    std::vector<std::pair<int, int>> pr(count);
+    std::vector<int> degree(H.n_atoms, 0);
+    for (size_t k = 0; k < count; ++k) {
+        pr[k] = {hi[k] - H.atom_begin, hj[k] - H.atom_begin};
+        ++degree[pr[k].first];
+        ++degree[pr[k].second];
+    }
+    if (mode == PRISM_MD_PAIRS_REFERENCE)
+        for (int d : degree)
+            if (d >= 1024) return fail(PRISM_MD_E_CAPACITY, "reference neighbour row saturated (>= 1024)");
+    std::sort(pr.begin(), pr.end());
+    pairs_i_.resize(count);
+    pairs_j_.resize(count);
+    for (size_t k = 0; k < count; ++k) {
+        pairs_i_[k] = pr[k].first;
+        pairs_j_[k] = pr[k].second;
+    }
+    return PRISM_MD_OK;
+}
+
+int Engine::get_pairs(int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out) {
+    if (!n_out) return fail(PRISM_MD_E_INVALID, "n_out");
+    *n_out = pairs_i_.size();
+    if (cap < pairs_i_.size()) return PRISM_MD_E_CAPACITY;
+    if (!pairs_i_.empty() && (!i_out || !j_out)) return fail(PRISM_MD_E_INVALID, "output arrays");
+    std::copy(pairs_i_.begin(), pairs_i_.end(), i_out);
+    std::copy(pairs_j_.begin(), pairs_j_.end(), j_out);
+    return PRISM_MD_OK;
+}
+
+int Engine::count_pairs(int s, uint64_t *n) {
+    (void)s;
+    unsigned long long c = 0;
+    hipMemsetAsync(d_count_, 0, sizeof(c), stream_);
+    launch_count_pairs(P_, d_count_, stream_);
+    if (int rc = d2h(&c, d_count_, sizeof(c))) return rc;
+    *n = c;
+    return PRISM_MD_OK;
+}
+
+int Engine::time_nonbonded(int iters, double *mean_ms) {
+    hipEvent_t a, b;
+    if (int rc = hipcheck(hipEventCreate(&a), "event")) return rc;
+    if (int rc = hipcheck(hipEventCreate(&b), "event")) return rc;
+    launch_nonbonded(P_, false, stream_);  // warm
+    hipEventRecord(a, stream_);
+    for (int k = 0; k < iters; ++k) launch_nonbonded(P_, false, stream_);
+    hipEventRecord(b, stream_);
+    hipEventSynchronize(b);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *mean_ms = (double)ms / iters;
+    full_force();  // restore the carried F(x)
+    return check_status();
+}
+
+// FIRE minimisation of the potential energy over all systems (unit FIRE time,
+// dt 0.002 -> at most 0.02, steps limited to 0.1 A), then every X-H bond set
+// to its constraint length.  Velocities of the MD state are untouched.
+int Engine::minimize(int max_iter, double f_tol, double *e_final, double *f_max) {
+    if (max_iter < 0 || !(f_tol > 0.0)) return fail(PRISM_MD_E_INVALID, "minimize arguments");
+    hipMemsetAsync(d_vmin_, 0, sizeof(float4) * P_.n_atoms_arena, stream_);
+    const int nb = fire_partial_blocks(P_);
+    std::vector<double> part((size_t)nb * 4);
+    double dt = 0.002, alpha = 0.1, fmax = 0.0;
+    int n_pos = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        full_force();
+        launch_fire_partials(P_, d_vmin_, d_fire_, stream_);
+        if (int rc = d2h(part.data(), d_fire_, part.size() * sizeof(double))) return rc;
+        double pw = 0.0, vv = 0.0, ff = 0.0, fm2 = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            pw += part[4 * b];
+            vv += part[4 * b + 1];
+            ff += part[4 * b + 2];
+            fm2 = std::max(fm2, part[4 * b + 3]);
+        }
+        fmax = std::sqrt(fm2);
+        if (!std::isfinite(ff)) return fail(PRISM_MD_E_NONFINITE, "non-finite force during minimisation");
+        if (fmax < f_tol) break;
+        float cv = 0.0f, cf = 0.0f;
+        if (pw > 0.0) {
+            cv = (float)(1.0 - alpha);
+            cf = (float)(alpha * std::sqrt(vv) / std::sqrt(ff));
+            if (++n_pos > 5) {
+                dt = std::min(dt * 1.1, 0.02);
+                alpha *= 0.99;
+            }
+        } else {
+            n_pos = 0;
+            dt *= 0.5;
+            alpha = 0.1;
+        }
+        launch_fire_update(P_, d_vmin_, cv, cf, (float)dt, 0.1f, stream_);
+        launch_sort(P_, -1, stream_);
+        launch_rows(P_, stream_);
+    }
+    launch_snap_hydrogens(P_, stream_);
+    if (int rc = refresh_all()) return rc;
+    if (int rc = check_status()) return rc;
+    if (e_final) {
+        double e[3];
+        if (int rc = energies(0, e)) return rc;
+        *e_final = e[0];
+    }
+    if (f_max) *f_max = fmax;
+    return PRISM_MD_OK;
+}
+
+uint64_t Engine::steps_done(int s) {
+    SysState S{};
+    d2h(&S, P_.sst + s, sizeof(S));
+    return S.step;
+}
+
+int Engine::stats(int s, PrismMdStatsV1 *out) {
+    if (s < 0 || s >= (int)sys_.size() || !out) return fail(PRISM_MD_E_INVALID, "stats");
+    if (int rc = hipcheck(hipStreamSynchronize(stream_), "sync")) return rc;
+    SysState S;
+    if (int rc = d2h(&S, P_.sst + s, sizeof(S))) return rc;
+    const HostSys &H = sys_[s];
+    std::vector<int> rn(H.n_blocks);
+    if (int rc = d2h(rn.data(), P_.row_n + H.block_begin, rn.size() * sizeof(int))) return rc;
+    std::memset(out, 0, sizeof(*out));
+    out->steps_done = S.step;
+    out->pair_rebuilds = (uint64_t)S.n_rebuilds;
+    out->force_evaluations = force_evals_;
+    out->n_tiles = std::accumulate(rn.begin(), rn.end(), 0);
+    out->n_blocks = H.n_blocks;
+    out->status = S.status;
+    return PRISM_MD_OK;
+}
+
+int Engine::set_active(int s, bool active) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    desc_[s].active = active ? 1 : 0;
+    return hipcheck(hipMemcpyAsync(d_desc_ + s, &desc_[s], sizeof(SysDesc), hipMemcpyHostToDevice, stream_), "activate");
+}
+
+}  // namespace prism
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct PrismMdHandle {
+    prism::Engine eng;
+    std::string err;
+};
+
+// Message of the last failed prism_md_create on this thread (the handle is
+// gone by then); returned by prism_md_last_error(NULL).
+static thread_local std::string g_create_error = "no error";
+
+extern "C" {
+
+int prism_md_abi_version(void) { return PRISM_MD_ABI_VERSION; }
+
+int prism_md_default_params(PrismMdParamsV1 *p) {
+    if (!p) return PRISM_MD_E_INVALID;
+    std::memset(p, 0, sizeof(*p));
+    p->dt_ps = 0.004;
+    p->temperature_K = 300.0;
+    p->end_temperature_K = 300.0;
+    p->gamma_ps = 1.0;
+    p->cutoff_A = 10.0;
+    p->skin_A = 2.0;
+    p->rebuild_every = 20;
+    p->com_remove_every = 100;
+    p->seed = 0x50524953ull;
+    p->stream_id = 0;
+    p->constraint_max_iter = 100;
+    p->shake_tol = 1e-10;
+    p->rattle_tol = 1e-12;
+    p->equilibration_gamma_ps = 1000.0;
+    return PRISM_MD_OK;
+}
+
+int prism_md_create(const PrismMdTopologyV1 *topology, const PrismMdParamsV1 *params, void *hip_stream, PrismMdHandle **out) {
+    if (!topology || !params || !out) return PRISM_MD_E_INVALID;
+    *out = nullptr;
+    PrismMdHandle *h = new (std::nothrow) PrismMdHandle();
+    if (!h) return PRISM_MD_E_NOMEM;
+    const int rc = h->eng.init({prism::SystemSpec{topology, *params}}, hip_stream);
+    if (rc != PRISM_MD_OK) {
+        g_create_error = h->eng.last_error();
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return PRISM_MD_OK;
+}
+
+int prism_md_destroy(PrismMdHandle *h) {
+    delete h;
+    return PRISM_MD_OK;
+}
+
+int prism_md_set_state(PrismMdHandle *h, const float *x, const float *v, int flags) {
+    return h ? h->eng.set_state(0, x, v, flags) : PRISM_MD_E_INVALID;
+}
+int prism_md_get_state(PrismMdHandle *h, float *x, float *v, int flags) {
+    return h ? h->eng.get_state(0, x, v, flags) : PRISM_MD_E_INVALID;
+}
+int prism_md_build_pairs(PrismMdHandle *h, int mode) { return h ? h->eng.build_pairs(0, mode) : PRISM_MD_E_INVALID; }
+int prism_md_get_pairs(PrismMdHandle *h, int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out) {
+    return h ? h->eng.get_pairs(i_out, j_out, cap, n_out) : PRISM_MD_E_INVALID;
+}
+int prism_md_compute_forces(PrismMdHandle *h, unsigned term_mask, float *F_out, double *E_terms_out) {
+    return h ? h->eng.compute_forces(term_mask, 0, F_out, nullptr, E_terms_out) : PRISM_MD_E_INVALID;
+}
+int prism_md_compute_forces_fixed(PrismMdHandle *h, unsigned term_mask, int64_t *F_fixed_out) {
+    return h ? h->eng.compute_forces(term_mask, 0, nullptr, F_fixed_out, nullptr) : PRISM_MD_E_INVALID;
+}
+int prism_md_step(PrismMdHandle *h, uint64_t n_steps, const float *noise) {
+    return h ? h->eng.step(n_steps, noise) : PRISM_MD_E_INVALID;
+}
+int prism_md_energies(PrismMdHandle *h, double out[3]) { return h && out ? h->eng.energies(0, out) : PRISM_MD_E_INVALID; }
+int prism_md_get_stats(PrismMdHandle *h, PrismMdStatsV1 *stats) { return h ? h->eng.stats(0, stats) : PRISM_MD_E_INVALID; }
+const char *prism_md_last_error(PrismMdHandle *h) { return h ? h->eng.last_error() : g_create_error.c_str(); }
+int prism_md_count_pairs(PrismMdHandle *h, uint64_t *n) { return h && n ? h->eng.count_pairs(0, n) : PRISM_MD_E_INVALID; }
+int prism_md_minimize(PrismMdHandle *h, int max_iter, double f_tol, double *e_final, double *f_max) {
+    return h ? h->eng.minimize(max_iter, f_tol, e_final, f_max) : PRISM_MD_E_INVALID;
+}
+int prism_md_time_nonbonded(PrismMdHandle *h, int iters, double *mean_ms) {
+    return h && mean_ms && iters > 0 ? h->eng.time_nonbonded(iters, mean_ms) : PRISM_MD_E_INVALID;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// engine.hpp -- host engine of the MI355X classical MD step.
+//
+// Plays the role of NhsAmberFusedEngine (crates/prism-nhs/src/fused_engine.rs:6554):
+// lowers topologies, owns every device buffer, sequences the phase kernels on
+// one HIP stream.  One Engine packs S >= 1 independent systems into shared
+// arenas (md_layout.hpp); the single-system C API is the S = 1 case.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/prism_md.h"
+#include "kernels.hpp"
+
+namespace prism {
+
+struct SystemSpec {
+    const PrismMdTopologyV1 *topo;
+    PrismMdParamsV1 params;
+};
+
+class Engine {
+  public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // Capacity in atoms / groups / terms is fixed at construction; systems may
+    // be (de)activated later through set_active.
+    int init(const std::vector<SystemSpec> &systems, void *stream);
+
+    int n_systems() const { return (int)sys_.size(); }
+    int n_atoms(int s) const { return sys_[s].n_atoms; }
+
+    int set_state(int s, const float *x, const float *v, int flags);
+    int get_state(int s, float *x, float *v, int flags);
+    int compute_forces(unsigned mask, int s, float *F_out, int64_t *F_fixed, double *E_terms);
+    int step(uint64_t n_steps, const float *noise);  // noise: single-system explicit array or NULL
+    int energies(int s, double out[3]);
+    int build_pairs(int s, int mode);
+    int get_pairs(int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out);
+    int count_pairs(int s, uint64_t *n);
+    int time_nonbonded(int iters, double *mean_ms);
+    int minimize(int max_iter, double f_tol, double *e_final, double *f_max);
+    int stats(int s, PrismMdStatsV1 *out);
+    int set_active(int s, bool active);
+    int check_status();           // synchronise, read device status, map to rc
+    const char *last_error() const { return err_.c_str(); }
+    hipStream_t stream() const { return stream_; }
+    uint64_t steps_done(int s);
+
+    // graph-captured stepping (batch path); n_steps must be a multiple of
+    // graph_steps once the graph exists
+    int step_graph(uint64_t n_steps, int graph_steps);
+
+  private:
+    int fail(int rc, const std::string &msg);
+    int hipcheck(hipError_t e, const char *what);
+    int d2h(void *dst, const void *src, size_t bytes);  // stream-ordered copy + synchronise
+    int refresh_all();            // rebuild lists + forces at current positions
+    void launch_step(const float *noise_n, const float *noise_np1);
+    void full_force();
+
+    struct HostSys {
+        int atom_begin, n_atoms, n_slots;
+        int block_begin, n_blocks;
+        int wave_begin, n_waves;
+        std::vector<float> x0;    // creation-time positions (reference grid)
+        RefGrid grid;
+        int grid_ok;
+        double dof;
+    };
+    std::vector<HostSys> sys_;
+    std::vector<SysDesc> desc_;
+    DevPtrs P_{};
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    std::vector<void *> allocs_;
+    bool alloc_failed_ = false;
+    std::string err_;
+    // host-side pair list of the last build_pairs
+    std::vector<int32_t> pairs_i_, pairs_j_;
+    // device scratch for diagnostics
+    unsigned long long *d_count_ = nullptr;
+    double *d_red_ = nullptr;
+    float *d_noise_ = nullptr;    // explicit noise staging [2][A][3]
+    float4 *d_vmin_ = nullptr;    // FIRE velocities
+    double *d_fire_ = nullptr;    // FIRE partials
+    SysDesc *d_desc_ = nullptr;
+    hipGraphExec_t graph_ = nullptr;
+    int graph_steps_ = 0;
+    uint64_t host_steps_ = 0;
+    uint64_t force_evals_ = 0;
+
+    template <class T> T *dalloc(size_t n);
+    template <class T> int upload(T *d, const std::vector<T> &h);
+};
+
+}  // namespace prism

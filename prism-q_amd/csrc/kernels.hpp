@@ -1,0 +1,76 @@
+// kernels.hpp -- device pointer bundle and launch wrappers of the MD kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/prism_md.h"
+#include "md_layout.hpp"
+
+namespace prism {
+
+struct DevPtrs {
+    // atoms, arena (topology) order
+    float4 *x, *v, *xs, *x_build;
+    long long *f;                   // [A][4] fixed-point force accumulators
+    const float *mass, *charge, *sigma, *eps;
+    const int *atom_sys;            // system of each arena atom (-1 padding)
+    // sorted slots (same arena index space)
+    int *perm;                      // slot -> atom, -1 padding
+    int *slot_of;                   // atom -> slot
+    float4 *xq;                     // sorted (x, y, z, q sqrt(332.0636))
+    float2 *lj;                     // sorted (sigma/2, sqrt(24 eps))
+    // blocks
+    float4 *bb_c, *bb_h;            // build-time box centre / half extent
+    const int *block_sys;           // system of each block (-1 padding)
+    int *row_n, *row_j, *row_m;     // [B], [B][MAX_ROW], [B][MAX_ROW]
+    int *unit_count;                // [B] nonbonded work units of each block's row
+    int *units;                     // [B * UNITS_PER_BLOCK] (block << 8 | quarter << 6 | chunk)
+    unsigned long long *masks;      // [B][MAX_MASKS][64]
+    // partner lists over arena atoms
+    const int *skip_off, *skip_idx; // exclusions U exception partners (nonbonded skip)
+    const int *ex_off, *ex_idx;     // exclusions minus exception partners (pair list)
+    // constraint groups
+    const Group *groups;
+    const GroupParam *gparams;
+    const int *wave_sys;            // system of each group wave (-1 padding)
+    // terms
+    const Bond *bonds;
+    const Angle *angles;
+    const Dihedral *dih;
+    const Exception14 *exc;
+    int n_bonds, n_angles, n_dih, n_exc;
+    // systems
+    const SysDesc *sys;
+    SysState *sst;
+    int n_sys;
+    int n_atoms_arena, n_blocks_arena, n_waves_arena;
+    // scratch
+    unsigned int *keys_a, *keys_b;
+    int *vals_a, *vals_b;
+    double *com_partial;            // [waves][3]
+    EngineCounters *counters;
+};
+
+struct RefGrid {                    // reference cell grid of the pair-list parity mode
+    float ox, oy, oz;
+    int nx, ny, nz;
+};
+
+// step_offset: 0 inside a step (lists built at x_(s+1) after SHAKE of step s),
+// -1 for a rebuild between steps (lists built at x_s).
+void launch_sort(const DevPtrs &P, int step_offset, hipStream_t s);
+void launch_rows(const DevPtrs &P, hipStream_t s);
+void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s);
+void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
+void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);
+void launch_second_half(const DevPtrs &P, const float *noise, hipStream_t s);
+void launch_zero_forces(const DevPtrs &P, hipStream_t s);
+void launch_kinetic(const DevPtrs &P, double *ke_out, hipStream_t s);
+void launch_emit_pairs(const DevPtrs &P, int mode, RefGrid g, int *out_i, int *out_j, unsigned long long *count,
+                       unsigned long long cap, hipStream_t s);
+void launch_count_pairs(const DevPtrs &P, unsigned long long *count, hipStream_t s);
+int fire_partial_blocks(const DevPtrs &P);
+void launch_fire_partials(const DevPtrs &P, const float4 *vmin, double *part, hipStream_t s);
+void launch_fire_update(const DevPtrs &P, float4 *vmin, float cv, float cf, float dt, float max_step, hipStream_t s);
+void launch_snap_hydrogens(const DevPtrs &P, hipStream_t s);
+
+}  // namespace prism

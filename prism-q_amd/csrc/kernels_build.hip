@@ -1,0 +1,388 @@
+// kernels_build.hip -- pair-list rebuild: per-system Hilbert cell hash with a
+// stable LSD radix sort, 64-atom spatial blocks, cluster-pair tile rows and
+// exclusion bit masks.
+//
+// Replaces reset_cell_counts / build_cell_list / nhs_build_neighbor_list_warp
+// (nhs_amber_fused.cu:11690-11795, nhs_neighbor_warp.cuh:10-119) driven by
+// rebuild_neighbor_lists (fused_engine.rs:16615-16748).  The reference keeps a
+// per-atom row of <= 1024 neighbour ids at 12 A, rebuilt after every SHAKE; this
+// build keeps block-pair tiles at cutoff + skin and rebuilds when an atom has
+// moved more than skin/2 or after rebuild_every steps.
+#include "device_common.hpp"
+
+namespace prism {
+
+namespace {
+
+constexpr int SORT_THREADS = 1024;
+constexpr int SORT_WAVES = SORT_THREADS / 64;
+constexpr float SORT_CELL = 2.0f;  // Hilbert cell edge, A
+
+// 3-D Hilbert index of cell (x, y, z), `bits` bits per axis, after J. Skilling,
+// "Programming the Hilbert curve", AIP Conf. Proc. 707, 381 (2004): the
+// axes-to-transpose transform followed by bit interleaving.  Unlike Morton
+// order the curve has no jumps, so 64 consecutive atoms form compact blocks.
+__device__ inline unsigned int hilbert_key(unsigned int x, unsigned int y, unsigned int z, int bits) {
+    unsigned int X[3] = {x, y, z};
+    const unsigned int M = 1u << (bits - 1);
+    for (unsigned int Q = M; Q > 1; Q >>= 1) {
+        const unsigned int P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const unsigned int t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    unsigned int t = 0;
+    for (unsigned int Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    unsigned int key = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+    return key;
+}
+
+// One workgroup per system.  Runs only for systems whose rebuild flag is set.
+__global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int step_offset) {
+    const int s = blockIdx.x;
+    if (s >= P.n_sys) return;
+    const SysDesc &D = P.sys[s];
+    SysState &S = P.sst[s];
+    __shared__ int go;
+    __shared__ float red[6][SORT_WAVES];
+    __shared__ unsigned int hist[256];
+    __shared__ unsigned int running[256];
+    __shared__ unsigned int wave_cnt[SORT_WAVES][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        go = D.active && S.rebuild_flag;
+        S.rows_dirty = go;
+    }
+    __syncthreads();
+    if (!go) return;
+    const int n = D.n_atoms, a0 = D.atom_begin;
+
+    // ---- bounding box of the current positions
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int i = tid; i < n; i += SORT_THREADS) {
+        const float4 p = P.x[a0 + i];
+        lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+        hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+    }
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = wave_min(lo[c]);
+        hi[c] = wave_max(hi[c]);
+    }
+    if (lane == 0)
+        for (int c = 0; c < 3; ++c) { red[c][wave] = lo[c]; red[3 + c][wave] = hi[c]; }
+    __syncthreads();
+    int bits = 1;
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = 3.0e38f;
+        hi[c] = -3.0e38f;
+        for (int w = 0; w < SORT_WAVES; ++w) {
+            lo[c] = fminf(lo[c], red[c][w]);
+            hi[c] = fmaxf(hi[c], red[3 + c][w]);
+        }
+        const unsigned int cells = (unsigned int)fminf((hi[c] - lo[c]) * (1.0f / SORT_CELL), 1023.0f) + 1u;
+        bits = max(bits, 32 - __clz(cells));  // bits to hold cell indices 0..cells-1
+    }
+    bits = min(bits, D.sort_bits);
+    const unsigned int cmax = (1u << bits) - 1u;
+
+    // ---- keys: Hilbert index of 2 A cells relative to the box minimum
+    for (int i = tid; i < n; i += SORT_THREADS) {
+        const float4 p = P.x[a0 + i];
+        const unsigned int cx = min(cmax, (unsigned int)fmaxf(0.0f, (p.x - lo[0]) * (1.0f / SORT_CELL)));
+        const unsigned int cy = min(cmax, (unsigned int)fmaxf(0.0f, (p.y - lo[1]) * (1.0f / SORT_CELL)));
+        const unsigned int cz = min(cmax, (unsigned int)fmaxf(0.0f, (p.z - lo[2]) * (1.0f / SORT_CELL)));
+        P.keys_a[a0 + i] = hilbert_key(cx, cy, cz, bits);
+        P.vals_a[a0 + i] = i;
+    }
+    __syncthreads();
+
+    // ---- stable LSD radix sort, 8-bit digits, one workgroup
+    unsigned int *kin = P.keys_a + a0, *kout = P.keys_b + a0;
+    int *vin = P.vals_a + a0, *vout = P.vals_b + a0;
+    const int total_bits = 3 * bits;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int shift = 0; shift < total_bits; shift += 8) {
+        if (tid < 256) hist[tid] = 0;
+        for (int k = 0; k < 4; ++k) wave_cnt[wave][lane + 64 * k] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += SORT_THREADS) atomicAdd(&hist[(kin[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        if (tid == 0) {  // exclusive scan (256 entries, serial)
+            unsigned int acc = 0;
+            for (int d = 0; d < 256; ++d) {
+                const unsigned int c = hist[d];
+                running[d] = acc;
+                acc += c;
+            }
+        }
+        __syncthreads();
+        for (int base = 0; base < n; base += SORT_THREADS) {
+            const int i = base + tid;
+            const bool valid = i < n;
+            const unsigned int key = valid ? kin[i] : 0u;
+            const int val = valid ? vin[i] : 0;
+            const unsigned int d = (key >> shift) & 255u;
+            unsigned long long peers = __ballot(valid);
+            if (!valid) peers = ~peers;
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const unsigned long long m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            const int rank = __popcll(peers & lt);
+            if (valid && rank == 0) wave_cnt[wave][d] = (unsigned int)__popcll(peers);
+            __syncthreads();
+            if (tid < 256) {
+                unsigned int acc = running[tid];
+                for (int w = 0; w < SORT_WAVES; ++w) {
+                    const unsigned int c = wave_cnt[w][tid];
+                    wave_cnt[w][tid] = acc;  // becomes the wave's offset for digit tid
+                    acc += c;
+                }
+                running[tid] = acc;
+            }
+            __syncthreads();
+            if (valid) {
+                const unsigned int dst = wave_cnt[wave][d] + rank;
+                kout[dst] = key;
+                vout[dst] = val;
+            }
+            __syncthreads();
+            if (tid < 256)
+                for (int w = 0; w < SORT_WAVES; ++w) wave_cnt[w][tid] = 0;
+            __syncthreads();
+        }
+        unsigned int *tk = kin; kin = kout; kout = tk;
+        int *tv = vin; vin = vout; vout = tv;
+        __syncthreads();
+    }
+
+    // ---- gather into sorted slots
+    const int slot0 = D.block_begin * BLOCK;
+    const float qscale = (float)sqrt(COULOMB);
+    for (int i = tid; i < D.n_blocks * BLOCK; i += SORT_THREADS) {
+        const int slot = slot0 + i;
+        if (i < n) {
+            const int a = a0 + vin[i];
+            const float4 p = P.x[a];
+            P.perm[slot] = a;
+            P.slot_of[a] = slot;
+            P.xq[slot] = make_float4(p.x, p.y, p.z, P.charge[a] * qscale);
+            P.lj[slot] = make_float2(0.5f * P.sigma[a], sqrtf(24.0f * P.eps[a]));
+            P.x_build[a] = p;
+        } else {
+            P.perm[slot] = -1;
+            P.xq[slot] = make_float4(DUMMY_FAR + 64.0f * (float)i, DUMMY_FAR, DUMMY_FAR, 0.0f);
+            P.lj[slot] = make_float2(0.0f, 0.0f);
+        }
+    }
+    __syncthreads();
+
+    // ---- block bounding boxes over real atoms
+    for (int b = wave; b < D.n_blocks; b += SORT_WAVES) {
+        const int i = b * BLOCK + lane;
+        float l[3], h[3];
+        if (i < n) {
+            const float4 p = P.xq[slot0 + i];
+            l[0] = h[0] = p.x; l[1] = h[1] = p.y; l[2] = h[2] = p.z;
+        } else {
+            l[0] = l[1] = l[2] = 3.0e38f;
+            h[0] = h[1] = h[2] = -3.0e38f;
+        }
+        for (int c = 0; c < 3; ++c) {
+            l[c] = wave_min(l[c]);
+            h[c] = wave_max(h[c]);
+        }
+        if (lane == 0) {
+            P.bb_c[D.block_begin + b] = make_float4(0.5f * (l[0] + h[0]), 0.5f * (l[1] + h[1]), 0.5f * (l[2] + h[2]), 0.0f);
+            P.bb_h[D.block_begin + b] = make_float4(0.5f * (h[0] - l[0]), 0.5f * (h[1] - l[1]), 0.5f * (h[2] - l[2]), 0.0f);
+        }
+    }
+    if (tid == 0) {
+        S.rebuild_flag = 0;
+        S.rebuild_step = (long long)S.step + step_offset;
+        S.n_rebuilds += 1;
+        atomicExch(&P.counters->any_dirty, 1);
+    }
+}
+
+// One wave per i-block: the row of j-blocks whose build-time boxes lie within
+// the list cutoff (full list, both directions, ascending j), then the
+// exclusion masks of the row's tiles that hold a skipped pair.
+__device__ void build_row(const DevPtrs &P, int b, int wave, int lane, int (*row_lds)[MAX_ROW], int (*flag_lds)[MAX_ROW]) {
+    const int s = P.block_sys[b];
+    const SysDesc &D = P.sys[s];
+    SysState &S = P.sst[s];
+    const float4 ci = P.bb_c[b], hi = P.bb_h[b];
+    const float reach = D.list_cutoff + 0.01f;
+    const float reach2 = reach * reach;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int count = 0;
+    const int jend = D.block_begin + D.n_blocks;
+    for (int j0 = D.block_begin; j0 < jend; j0 += 64) {
+        const int jb = j0 + lane;
+        bool ok = false;
+        if (jb < jend) {
+            const float4 cj = P.bb_c[jb], hj = P.bb_h[jb];
+            const float dx = fmaxf(fabsf(ci.x - cj.x) - hi.x - hj.x, 0.0f);
+            const float dy = fmaxf(fabsf(ci.y - cj.y) - hi.y - hj.y, 0.0f);
+            const float dz = fmaxf(fabsf(ci.z - cj.z) - hi.z - hj.z, 0.0f);
+            ok = dx * dx + dy * dy + dz * dz <= reach2;
+        }
+        const unsigned long long m = __ballot(ok);
+        const int pos = count + __popcll(m & lt);
+        if (ok && pos < MAX_ROW) {
+            P.row_j[(size_t)b * MAX_ROW + pos] = jb;
+            row_lds[wave][pos] = jb;
+        }
+        count += __popcll(m);
+    }
+    if (count > MAX_ROW) {
+        if (lane == 0) atomicOr(&S.status, ST_ROW_OVERFLOW);
+        count = MAX_ROW;
+    }
+    if (lane == 0) {
+        P.row_n[b] = count;
+        P.unit_count[b] = QUARTERS * ((count + CHUNK - 1) / CHUNK);
+    }
+    for (int t = lane; t < MAX_ROW; t += 64) flag_lds[wave][t] = 0;
+    wave_lds_fence();
+
+    // tiles holding skipped pairs (and the diagonal tile, for the self pair)
+    const int slot = b * BLOCK + lane;
+    const int a = P.perm[slot];
+    auto find = [&](int jb) -> int {
+        int lo = 0, hi2 = count - 1;
+        while (lo <= hi2) {
+            const int mid = (lo + hi2) >> 1;
+            const int v = row_lds[wave][mid];
+            if (v == jb) return mid;
+            if (v < jb) lo = mid + 1; else hi2 = mid - 1;
+        }
+        return -1;
+    };
+    const int tdiag = find(b);
+    if (lane == 0 && tdiag >= 0) flag_lds[wave][tdiag] = 1;
+    if (a >= 0)
+        for (int e = P.skip_off[a]; e < P.skip_off[a + 1]; ++e) {
+            const int t = find(P.slot_of[P.skip_idx[e]] / BLOCK);
+            if (t >= 0) atomicOr(&flag_lds[wave][t], 1);
+        }
+    wave_lds_fence();
+    int n_masks = 0;
+    for (int t0 = 0; t0 < count; t0 += 64) {
+        const int t = t0 + lane;
+        const bool f = t < count && flag_lds[wave][t];
+        const unsigned long long m = __ballot(f);
+        const int ms = n_masks + __popcll(m & lt);
+        if (t < count) {
+            const int idx = (f && ms < MAX_MASKS) ? ms : -1;
+            P.row_m[(size_t)b * MAX_ROW + t] = idx;
+            flag_lds[wave][t] = idx;  // tile -> mask slot
+        }
+        n_masks += __popcll(m);
+    }
+    if (n_masks > MAX_MASKS && lane == 0) atomicOr(&S.status, ST_MASK_OVERFLOW);
+    wave_lds_fence();
+    unsigned long long *mrow = P.masks + (size_t)b * MAX_MASKS * 64;
+    for (int m = 0; m < min(n_masks, MAX_MASKS); ++m) mrow[m * 64 + lane] = 0ull;
+    if (tdiag >= 0) {
+        const int ms = flag_lds[wave][tdiag];
+        if (ms >= 0) mrow[ms * 64 + lane] = 1ull << lane;  // self pair
+    }
+    if (a >= 0)
+        for (int e = P.skip_off[a]; e < P.skip_off[a + 1]; ++e) {
+            const int ps = P.slot_of[P.skip_idx[e]];
+            const int t = find(ps / BLOCK);
+            if (t < 0) continue;
+            const int ms = flag_lds[wave][t];
+            if (ms >= 0) mrow[ms * 64 + lane] |= 1ull << (ps % BLOCK);
+        }
+}
+
+constexpr int ROW_WAVES = 4;
+__global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * ROW_WAVES + wave;
+    __shared__ int row_lds[ROW_WAVES][MAX_ROW];
+    __shared__ int flag_lds[ROW_WAVES][MAX_ROW];
+    __shared__ int last;
+    if (b < P.n_blocks_arena) {
+        const int s = P.block_sys[b];
+        if (s >= 0 && P.sys[s].active && P.sst[s].rows_dirty) build_row(P, b, wave, lane, row_lds, flag_lds);
+    }
+    // Last-block handshake (no waiting): when any row changed, the workgroup
+    // that retires last rebuilds the compact list of nonbonded work units,
+    // block after block in arena order.
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&P.counters->rows_done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    __shared__ int dirty, base;
+    if (threadIdx.x == 0) {
+        dirty = P.counters->any_dirty;
+        base = 0;
+    }
+    __syncthreads();
+    if (dirty) {
+        for (int b0 = 0; b0 < P.n_blocks_arena; b0 += 64 * ROW_WAVES) {
+            const int bb = b0 + threadIdx.x;
+            int cnt = 0;
+            if (bb < P.n_blocks_arena) {
+                const int s = P.block_sys[bb];
+                cnt = (s >= 0 && P.sys[s].active) ? P.unit_count[bb] : 0;
+            }
+            // workgroup exclusive scan of cnt
+            __shared__ int scan[64 * ROW_WAVES];
+            scan[threadIdx.x] = cnt;
+            __syncthreads();
+            for (int o = 1; o < 64 * ROW_WAVES; o <<= 1) {
+                const int v = threadIdx.x >= o ? scan[threadIdx.x - o] : 0;
+                __syncthreads();
+                scan[threadIdx.x] += v;
+                __syncthreads();
+            }
+            const int off = base + scan[threadIdx.x] - cnt;
+            const int per_q = cnt / QUARTERS;  // chunks per quarter
+            for (int k = 0; k < cnt; ++k) {
+                const int q = k / per_q, c = k % per_q;
+                P.units[off + k] = (bb << 8) | (q << 6) | c;
+            }
+            __syncthreads();
+            if (threadIdx.x == 64 * ROW_WAVES - 1) base = off + cnt;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            P.counters->n_units = base;
+            P.counters->any_dirty = 0;
+        }
+    }
+    if (threadIdx.x == 0) P.counters->rows_done = 0;
+}
+
+}  // namespace
+
+void launch_sort(const DevPtrs &P, int step_offset, hipStream_t st) {
+    hipLaunchKernelGGL(k_sort, dim3(P.n_sys), dim3(SORT_THREADS), 0, st, P, step_offset);
+}
+
+void launch_rows(const DevPtrs &P, hipStream_t st) {
+    const int grid = (P.n_blocks_arena + ROW_WAVES - 1) / ROW_WAVES;
+    hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), 0, st, P);
+}
+
+}  // namespace prism

@@ -1,0 +1,399 @@
+// kernels_force.hip -- force evaluation: nonbonded cluster-pair tiles,
+// bonded terms and exact 1-4 exceptions, all accumulated into int64 fixed
+// point (2^24 per kcal/mol/A) so the summed force does not depend on the
+// order in which tiles and terms retire.
+//
+// Replaces the FORCE_N / FORCE_NP1 gather of prism_amber_phase_body_v1
+// (nhs_amber_fused.cu:2981-3484) with its term functions compute_bond_force
+// (:354-386), compute_angle_force (:388-448), compute_dihedral_force
+// (:524-625), compute_nonbonded_force (:643-717) and
+// ordinary_cutoff_potential_offset_v1 (:631-640).
+#include "device_common.hpp"
+
+namespace prism {
+
+namespace {
+
+constexpr int NB_WAVES = 4;
+
+// Cutoff test of the nonbonded pair: r2 = fma(dz, dz, fma(dy, dy, dx*dx)) in
+// FP32 with d = x_i - x_j.  The oracle decides pair membership with the same
+// rounding (fmaf), so the evaluated pair set is identical on both sides.
+__device__ inline float pair_r2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
+
+// Bit patterns of positive floats order like the floats: one unsigned compare
+// tests 0.01 <= r2 < cutoff2 (the reference's skip, nhs_amber_fused.cu:3159).
+__device__ inline bool in_shell(float r2, unsigned lo_bits, unsigned span) {
+    return (unsigned)(__float_as_uint(r2) - lo_bits) < span;
+}
+
+template <bool ENERGY, bool MASKED>
+__device__ inline void tile_pairs(const float4 xi, const float2 li, int n_act, int js, const float4 *jpos,
+                                  const float4 *jpar, unsigned long long excl, unsigned lo_bits, unsigned span,
+                                  float inv_rc, float inv_rc2, float &fx, float &fy, float &fz, float &e) {
+    const int n_iter = (n_act + 3) >> 2;
+#pragma unroll 2
+    for (int k = 0; k < n_iter; ++k) {
+        const int jj = 4 * k + js;
+        const bool valid = jj < n_act;
+        const float4 q = jpos[valid ? jj : 0];
+        const float4 pr = jpar[valid ? jj : 0];
+        const float dx = xi.x - q.x, dy = xi.y - q.y, dz = xi.z - q.z;
+        const float r2 = pair_r2(dx, dy, dz);
+        bool keep = valid && in_shell(r2, lo_bits, span);
+        if (MASKED) keep = keep && !((excl >> __float_as_int(pr.z)) & 1ull);
+        const float inv_r = rsqrtf(r2);
+        const float inv_r2 = inv_r * inv_r;
+        const float sig = li.x + pr.x;
+        const float s2 = sig * sig * inv_r2;
+        const float s6 = s2 * s2 * s2;
+        const float eps24 = li.y * pr.y;  // 24 sqrt(eps_i eps_j)
+        const float fc = (xi.w * q.w) * inv_r;  // 332.0636 q_i q_j / r
+        float f = fmaf(eps24 * s6, fmaf(2.0f, s6, -1.0f), fc) * inv_r2;
+        f = keep ? f : 0.0f;
+        fx = fmaf(f, dx, fx);
+        fy = fmaf(f, dy, fy);
+        fz = fmaf(f, dz, fz);
+        if (ENERGY) {
+            const float c2 = sig * sig * inv_rc2;
+            const float c6 = c2 * c2 * c2;
+            const float v = (eps24 * (1.0f / 6.0f)) * ((s6 * s6 - s6) - (c6 * c6 - c6)) + (xi.w * q.w) * (inv_r - inv_rc);
+            e += keep ? 0.5f * v : 0.0f;
+        }
+    }
+}
+
+// Persistent grid over the compact unit list.  A unit is (i-block b, quarter
+// q, chunk c of CHUNK tiles of b's row).  Lane l works on i-atom
+// b*64 + q*16 + (l & 15) against j-slot l >> 4: every instruction evaluates
+// 16 i-atoms x 4 j-atoms.  Per tile the 64 j-atoms are culled against the
+// quarter's current bounding box at the cutoff and the survivors compacted
+// into LDS.  The row is a full list (both directions), so each lane only
+// accumulates its own i-force; the four j-slot partials are summed at the end
+// of the unit and added once to the int64 fixed-point accumulator.
+template <bool ENERGY>
+__global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int il = lane & (SUB - 1), js = lane >> 4;
+    __shared__ float4 jpos_s[NB_WAVES][64];
+    __shared__ float4 jpar_s[NB_WAVES][64];
+    float4 *jpos = jpos_s[wave];
+    float4 *jpar = jpar_s[wave];
+    const int n_units = P.counters->n_units;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int u = blockIdx.x * NB_WAVES + wave; u < n_units; u += gridDim.x * NB_WAVES) {
+        const int code = P.units[u];
+        const int b = code >> 8, q = (code >> 6) & 3, c = code & 63;
+        const int s = P.block_sys[b];
+        const SysDesc &D = P.sys[s];
+        const int nrow = P.row_n[b];
+        const int t0 = c * CHUNK, t1 = min(nrow, t0 + CHUNK);
+        const float reach = D.cutoff + 0.001f;
+        const float reach2 = reach * reach;
+        const unsigned lo_bits = __float_as_uint(0.01f);
+        const unsigned span = __float_as_uint(D.cutoff2) - lo_bits;
+        const float inv_rc2 = 1.0f / D.cutoff2, inv_rc = rsqrtf(D.cutoff2);
+
+        const int slot_i = b * BLOCK + q * SUB + il;
+        const float4 xi = P.xq[slot_i];
+        const float2 li = P.lj[slot_i];
+        const int ai = P.perm[slot_i];
+        float lo[3], hi[3];
+        if (ai >= 0) {
+            lo[0] = hi[0] = xi.x; lo[1] = hi[1] = xi.y; lo[2] = hi[2] = xi.z;
+        } else {
+            lo[0] = lo[1] = lo[2] = 3.0e38f;
+            hi[0] = hi[1] = hi[2] = -3.0e38f;
+        }
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = wave_min(lo[k]);
+            hi[k] = wave_max(hi[k]);
+        }
+        const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
+        const float hx = 0.5f * (hi[0] - lo[0]), hy = 0.5f * (hi[1] - lo[1]), hz = 0.5f * (hi[2] - lo[2]);
+
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
+        for (int t = t0; t < t1; ++t) {
+            const int jb = P.row_j[(size_t)b * MAX_ROW + t];
+            const int ms = P.row_m[(size_t)b * MAX_ROW + t];
+            const float4 xj = P.xq[jb * BLOCK + lane];
+            const float2 lj = P.lj[jb * BLOCK + lane];
+            const float ex = fmaxf(fabsf(xj.x - cx) - hx, 0.0f);
+            const float ey = fmaxf(fabsf(xj.y - cy) - hy, 0.0f);
+            const float ez = fmaxf(fabsf(xj.z - cz) - hz, 0.0f);
+            const bool active = ex * ex + ey * ey + ez * ez <= reach2;
+            const unsigned long long m = __ballot(active);
+            const int n_act = __popcll(m);
+            if (n_act == 0) continue;
+            if (active) {
+                const int pos = __popcll(m & lt);
+                jpos[pos] = xj;
+                jpar[pos] = make_float4(lj.x, lj.y, __int_as_float(lane), 0.0f);
+            }
+            wave_lds_fence();
+            if (ms >= 0) {
+                const unsigned long long excl = P.masks[((size_t)b * MAX_MASKS + ms) * 64 + q * SUB + il];
+                tile_pairs<ENERGY, true>(xi, li, n_act, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
+            } else {
+                tile_pairs<ENERGY, false>(xi, li, n_act, js, jpos, jpar, 0ull, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
+            }
+            wave_lds_fence();
+        }
+        // sum the four j-slot partials of each i-atom (lanes il, il+16, il+32, il+48)
+        fx += __shfl_xor(fx, 16); fy += __shfl_xor(fy, 16); fz += __shfl_xor(fz, 16);
+        fx += __shfl_xor(fx, 32); fy += __shfl_xor(fy, 32); fz += __shfl_xor(fz, 32);
+        if (js == 0 && ai >= 0) {
+            add_fixed(&P.f[(size_t)ai * 4 + 0], fx, FORCE_FIXED_BITS);
+            add_fixed(&P.f[(size_t)ai * 4 + 1], fy, FORCE_FIXED_BITS);
+            add_fixed(&P.f[(size_t)ai * 4 + 2], fz, FORCE_FIXED_BITS);
+        }
+        if (ENERGY) {
+            const double es = wave_sum((double)e);
+            if (lane == 0)
+                atomicAdd((unsigned long long *)&P.sst[s].energy[3],
+                          (unsigned long long)(long long)rint(es * (double)(1ll << ENERGY_FIXED_BITS)));
+        }
+    }
+}
+
+__device__ inline float3 ld3(const float4 *x, int a) {
+    const float4 p = x[a];
+    return make_float3(p.x, p.y, p.z);
+}
+__device__ inline double3 ld3d(const float4 *x, int a) {
+    const float4 p = x[a];
+    return make_double3(p.x, p.y, p.z);
+}
+__device__ inline void add_fixed_d(long long *acc, double v) {
+    double d = v * (double)(1ll << FORCE_FIXED_BITS);
+    d = fmin(fmax(d, -4.0e18), 4.0e18);
+    atomicAdd((unsigned long long *)acc, (unsigned long long)(long long)rint(d));
+}
+__device__ inline void add_force(const DevPtrs &P, int a, double x, double y, double z) {
+    add_fixed_d(&P.f[(size_t)a * 4 + 0], x);
+    add_fixed_d(&P.f[(size_t)a * 4 + 1], y);
+    add_fixed_d(&P.f[(size_t)a * 4 + 2], z);
+}
+__device__ inline void add_energy(const DevPtrs &P, int atom, int term, double v) {
+    const int s = P.atom_sys[atom];
+    atomicAdd((unsigned long long *)&P.sst[s].energy[term],
+              (unsigned long long)(long long)rint(v * (double)(1ll << ENERGY_FIXED_BITS)));
+}
+__device__ inline double3 sub3(double3 a, double3 b) { return make_double3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline double dot3(double3 a, double3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline double3 cross3(double3 a, double3 b) {
+    return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// One thread per term: bonds, angles, dihedrals, exceptions in that order.
+// The bonded terms and exceptions evaluate in FP64 from the FP32 state: on
+// CDNA4 the FP64 vector rate equals the unpacked FP32 rate, and it removes the
+// cancellation of r - r0 and theta - theta0 near equilibrium that limits an
+// FP32 evaluation to ~1e-5 relative.
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < P.n_bonds) {
+        if (!(mask & PRISM_MD_TERM_BOND)) return;
+        const Bond B = P.bonds[idx];
+        if (B.i < 0) return;
+        const double3 r = sub3(ld3d(P.x, B.j), ld3d(P.x, B.i));
+        const double len = sqrt(dot3(r, r));
+        if (len < 1e-8) return;
+        const double dr = len - (double)B.r0;
+        const double fm = -2.0 * (double)B.k * dr / len;
+        add_force(P, B.i, -fm * r.x, -fm * r.y, -fm * r.z);
+        add_force(P, B.j, fm * r.x, fm * r.y, fm * r.z);
+        if (ENERGY) add_energy(P, B.i, 0, (double)B.k * dr * dr);
+        return;
+    }
+    idx -= P.n_bonds;
+    if (idx < P.n_angles) {
+        if (!(mask & PRISM_MD_TERM_ANGLE)) return;
+        const Angle A = P.angles[idx];
+        if (A.i < 0) return;
+        const double3 pj = ld3d(P.x, A.j);
+        const double3 a = sub3(ld3d(P.x, A.i), pj), b = sub3(ld3d(P.x, A.k), pj);
+        const double la = sqrt(dot3(a, a)), lb = sqrt(dot3(b, b));
+        if (la < 1e-8 || lb < 1e-8) return;
+        double cs = dot3(a, b) / (la * lb);
+        cs = fmax(-1.0, fmin(1.0, cs));
+        const double dth = acos(cs) - (double)A.theta0;
+        double sn = sqrt(1.0 - cs * cs);
+        if (sn < 1e-8) sn = 1e-8;
+        const double fm = -2.0 * (double)A.kf * dth / sn;
+        const double ia = 1.0 / la, ib = 1.0 / lb;
+        const double3 di = make_double3((b.x * ib - cs * a.x * ia) * ia, (b.y * ib - cs * a.y * ia) * ia, (b.z * ib - cs * a.z * ia) * ia);
+        const double3 dk = make_double3((a.x * ia - cs * b.x * ib) * ib, (a.y * ia - cs * b.y * ib) * ib, (a.z * ia - cs * b.z * ib) * ib);
+        add_force(P, A.i, -fm * di.x, -fm * di.y, -fm * di.z);
+        add_force(P, A.k, -fm * dk.x, -fm * dk.y, -fm * dk.z);
+        add_force(P, A.j, fm * (di.x + dk.x), fm * (di.y + dk.y), fm * (di.z + dk.z));
+        if (ENERGY) add_energy(P, A.i, 1, (double)A.kf * dth * dth);
+        return;
+    }
+    idx -= P.n_angles;
+    if (idx < P.n_dih) {
+        if (!(mask & PRISM_MD_TERM_DIHEDRAL)) return;
+        const Dihedral Q = P.dih[idx];
+        if (Q.i < 0) return;
+        const double3 pj = ld3d(P.x, Q.j), pk = ld3d(P.x, Q.k);
+        const double3 v0 = sub3(ld3d(P.x, Q.i), pj), v1 = sub3(pk, pj), v2 = sub3(pk, ld3d(P.x, Q.l));
+        const double3 c0 = cross3(v0, v1), c1 = cross3(v1, v2);
+        const double n1 = dot3(c0, c0), n2 = dot3(c1, c1), nb = dot3(v1, v1);
+        if (n1 < 1e-16 || n2 < 1e-16 || nb < 1e-16) return;
+        const double lb = sqrt(nb);
+        // phi from atan2 of |c0 x c1| and c0.c1: the reference's acos / asin
+        // branches (nhs_amber_fused.cu:562-580) in exact arithmetic
+        const double3 cc = cross3(c0, c1);
+        double phi = atan2(sqrt(dot3(cc, cc)), dot3(c0, c1));
+        if (dot3(v0, c1) < 0.0) phi = -phi;
+        const double nn = (double)Q.n;
+        const double dE = -(double)Q.kf * nn * sin(nn * phi - (double)Q.phase);
+        const double ff0 = (-dE * lb) / n1, ff3 = (dE * lb) / n2;
+        const double ff1 = dot3(v0, v1) / nb, ff2 = dot3(v2, v1) / nb;
+        const double3 f1 = make_double3(ff0 * c0.x, ff0 * c0.y, ff0 * c0.z);
+        const double3 f4 = make_double3(ff3 * c1.x, ff3 * c1.y, ff3 * c1.z);
+        const double3 sv = make_double3(ff1 * f1.x - ff2 * f4.x, ff1 * f1.y - ff2 * f4.y, ff1 * f1.z - ff2 * f4.z);
+        add_force(P, Q.i, f1.x, f1.y, f1.z);
+        add_force(P, Q.j, sv.x - f1.x, sv.y - f1.y, sv.z - f1.z);
+        add_force(P, Q.k, -sv.x - f4.x, -sv.y - f4.y, -sv.z - f4.z);
+        add_force(P, Q.l, f4.x, f4.y, f4.z);
+        if (ENERGY) add_energy(P, Q.i, 2, (double)Q.kf * (1.0 + cos(nn * phi - (double)Q.phase)));
+        return;
+    }
+    idx -= P.n_dih;
+    if (idx < P.n_exc) {
+        if (!(mask & PRISM_MD_TERM_EXCEPTION)) return;
+        const Exception14 X = P.exc[idx];
+        if (X.i < 0) return;
+        const double3 d = sub3(ld3d(P.x, X.i), ld3d(P.x, X.j));
+        const double r2 = dot3(d, d);
+        if (r2 < 1e-8) {  // coincident exception atoms (nhs_amber_fused.cu:3338-3343)
+            atomicOr(&P.sst[P.atom_sys[X.i]].status, ST_EXCEPTION_COINCIDENT);
+            return;
+        }
+        const double inv_r2 = 1.0 / r2, inv_r = sqrt(inv_r2);
+        const double s2 = (double)X.sigma * (double)X.sigma * inv_r2, s6 = s2 * s2 * s2;
+        const double qq = COULOMB * (double)X.qq;
+        const double eps = X.eps;
+        const double f = (24.0 * eps * s6 * (2.0 * s6 - 1.0) + qq * inv_r) * inv_r2;
+        add_force(P, X.i, f * d.x, f * d.y, f * d.z);
+        add_force(P, X.j, -f * d.x, -f * d.y, -f * d.z);
+        if (ENERGY) add_energy(P, X.i, 4, 4.0 * eps * (s6 * s6 - s6) + qq * inv_r);
+    }
+}
+
+__global__ void k_zero_forces(DevPtrs P) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (size_t)P.n_atoms_arena * 4) P.f[i] = 0;
+}
+
+__device__ inline float r2_list(float3 a, float3 b) {  // (dx*dx + dy*dy) + dz*dz, dx = b - a, no contraction
+    const float dx = __fsub_rn(b.x, a.x), dy = __fsub_rn(b.y, a.y), dz = __fsub_rn(b.z, a.z);
+    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+__device__ inline bool in_list(const int *off, const int *idx, int a, int b) {
+    for (int e = off[a]; e < off[a + 1]; ++e)
+        if (idx[e] == b) return true;
+    return false;
+}
+__device__ inline int ref_cell(float x, float o, int n) {  // nhs_amber_fused.cu:11704-11712
+    const int c = (int)__fmul_rn(__fsub_rn(x, o), 0.1f);
+    return max(0, min(c, n - 1));
+}
+
+// Pair enumeration over the tile rows.  mode 0 / 1: the parity pair list
+// (reference cell scheme / brute force) -- unique pairs a_i < a_j with FP32
+// r2 <= 144 that are not 1-2/1-3 exclusions; mode 2: count the pairs the
+// nonbonded kernel evaluates (r2 < cutoff^2, r2 >= 0.01, not skipped).
+__global__ __launch_bounds__(256) void k_pairs(DevPtrs P, int mode, RefGrid g, int *out_i, int *out_j,
+                                               unsigned long long *count, unsigned long long cap) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + wave;
+    if (b >= P.n_blocks_arena) return;
+    const int s = P.block_sys[b];
+    if (s < 0 || !P.sys[s].active) return;
+    const int ai = P.perm[b * BLOCK + lane];
+    float3 xi = make_float3(0, 0, 0);
+    if (ai >= 0) xi = ld3(P.x, ai);
+    int gi[3] = {0, 0, 0};
+    if (mode == 0 && ai >= 0) {
+        gi[0] = ref_cell(xi.x, g.ox, g.nx);
+        gi[1] = ref_cell(xi.y, g.oy, g.ny);
+        gi[2] = ref_cell(xi.z, g.oz, g.nz);
+    }
+    const float cutoff2 = P.sys[s].cutoff2;
+    unsigned long long local = 0;
+    const int nrow = P.row_n[b];
+    for (int t = 0; t < nrow; ++t) {
+        const int jb = P.row_j[(size_t)b * MAX_ROW + t];
+        for (int k = 0; k < BLOCK; ++k) {
+            const int aj = P.perm[jb * BLOCK + k];
+            if (aj < 0 || ai < 0 || ai >= aj) continue;
+            const float3 xj = ld3(P.x, aj);
+            bool keep;
+            if (mode == 2) {
+                const float r2 = pair_r2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+                keep = r2 < cutoff2 && r2 >= 0.01f && !in_list(P.skip_off, P.skip_idx, ai, aj);
+            } else {
+                keep = !(r2_list(xi, xj) > 144.0f) && !in_list(P.ex_off, P.ex_idx, ai, aj);
+                if (keep && mode == 0) {
+                    keep = abs(ref_cell(xj.x, g.ox, g.nx) - gi[0]) <= 1 && abs(ref_cell(xj.y, g.oy, g.ny) - gi[1]) <= 1 &&
+                           abs(ref_cell(xj.z, g.oz, g.nz) - gi[2]) <= 1;
+                }
+            }
+            if (!keep) continue;
+            if (mode == 2) {
+                ++local;
+            } else {
+                const unsigned long long pos = atomicAdd(count, 1ull);
+                if (pos < cap) {
+                    out_i[pos] = ai;
+                    out_j[pos] = aj;
+                }
+            }
+        }
+    }
+    if (mode == 2) {
+        for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
+        if (lane == 0) atomicAdd(count, local);
+    }
+}
+
+}  // namespace
+
+void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st) {
+    // persistent grid: enough waves to cover the unit list of a typical row
+    // (80 tiles -> 40 units per block), capped at 16 waves per CU
+    const int want = (P.n_blocks_arena * QUARTERS * 12 + NB_WAVES - 1) / NB_WAVES;
+    const int grid = max(1, min(want, 256 * 16 / NB_WAVES));
+    if (energy) hipLaunchKernelGGL(k_nonbonded<true>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
+    else hipLaunchKernelGGL(k_nonbonded<false>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
+}
+
+void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t st) {
+    const int n = P.n_bonds + P.n_angles + P.n_dih + P.n_exc;
+    if (n == 0) return;
+    const int grid = (n + 255) / 256;
+    if (energy) hipLaunchKernelGGL(k_bonded<true>, dim3(grid), dim3(256), 0, st, P, term_mask);
+    else hipLaunchKernelGGL(k_bonded<false>, dim3(grid), dim3(256), 0, st, P, term_mask);
+}
+
+void launch_zero_forces(const DevPtrs &P, hipStream_t st) {
+    const size_t n = (size_t)P.n_atoms_arena * 4;
+    hipLaunchKernelGGL(k_zero_forces, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P);
+}
+
+void launch_emit_pairs(const DevPtrs &P, int mode, RefGrid g, int *out_i, int *out_j, unsigned long long *count,
+                       unsigned long long cap, hipStream_t st) {
+    const int grid = (P.n_blocks_arena + 3) / 4;
+    hipLaunchKernelGGL(k_pairs, dim3(grid), dim3(256), 0, st, P, mode, g, out_i, out_j, count, cap);
+}
+
+void launch_count_pairs(const DevPtrs &P, unsigned long long *count, hipStream_t st) {
+    const int grid = (P.n_blocks_arena + 3) / 4;
+    hipLaunchKernelGGL(k_pairs, dim3(grid), dim3(256), 0, st, P, 2, RefGrid{}, (int *)nullptr, (int *)nullptr, count,
+                       0ull);
+}
+
+}  // namespace prism

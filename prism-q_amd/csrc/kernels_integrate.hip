@@ -1,0 +1,338 @@
+// kernels_integrate.hip -- constrained Langevin OVRVO integrator.
+//
+// One lane per constraint group (a heavy atom with its <= 3 hydrogens, or a
+// lone atom).  Because every constraint couples atoms of one group only, all
+// per-atom and per-cluster phases between two force evaluations fuse into one
+// kernel; the reference launches each as a separate grid-wide phase
+// (fused_engine.rs:6223-6235, nhs_amber_fused.cu:3620-3850, 11128-11186):
+//   first half  (phases OU_HALF_N, RATTLE_AFTER_OU_N, KICK_N, RATTLE_PRE_DRIFT,
+//                DRIFT_SAVE_XN, SHAKE_POSITION), preceded by the COM removal of
+//                steps s > 0, s % com_every == 0 (fused_engine.rs:19023);
+//   second half (KICK_NP1, RATTLE_BEFORE_OU_NP1, OU_HALF_NP1, velocity safety
+//                clamp, final RATTLE).
+// Kicks, Ornstein-Uhlenbeck updates and drifts are FP32 as in the reference;
+// SHAKE and RATTLE iterate in FP64 on the group's atoms.
+#include "device_common.hpp"
+
+namespace prism {
+
+namespace {
+
+constexpr int IW = 4;  // waves per workgroup
+
+struct GroupState {
+    int n;              // atoms in the group (1..4)
+    int a[4];
+    float m[4];
+    float3 x[4], v[4];
+};
+
+__device__ inline float3 f3(float4 p) { return make_float3(p.x, p.y, p.z); }
+
+// rattle_cluster_velocities (nhs_amber_fused.cu:2175-2360), FP64 iteration.
+__device__ bool rattle(GroupState &g, const GroupParam &gp, double tol, int max_iter) {
+    if (g.n < 2) return true;
+    double v[4][3], x[4][3];
+    for (int k = 0; k < g.n; ++k) {
+        x[k][0] = g.x[k].x; x[k][1] = g.x[k].y; x[k][2] = g.x[k].z;
+        v[k][0] = g.v[k].x; v[k][1] = g.v[k].y; v[k][2] = g.v[k].z;
+    }
+    const double imc = gp.inv_mass_c, imh = gp.inv_mass_h;
+    bool converged = false;
+    for (int it = 0; it < max_iter && !converged; ++it) {
+        double max_err = 0.0;
+        for (int h = 1; h < g.n; ++h) {
+            const double rx = x[h][0] - x[0][0], ry = x[h][1] - x[0][1], rz = x[h][2] - x[0][2];
+            const double r2 = rx * rx + ry * ry + rz * rz;
+            if (!(r2 >= 1e-12) || !isfinite(r2)) return false;
+            const double ux = v[h][0] - v[0][0], uy = v[h][1] - v[0][1], uz = v[h][2] - v[0][2];
+            const double res = rx * ux + ry * uy + rz * uz;
+            const double rel = fabs(res) / (sqrt(r2) * fmax(sqrt(ux * ux + uy * uy + uz * uz), 1.0));
+            if (!isfinite(rel)) return false;
+            max_err = fmax(max_err, rel);
+            if (rel <= tol) continue;
+            const double lam = res / ((imc + imh) * r2);
+            v[0][0] += imc * lam * rx; v[0][1] += imc * lam * ry; v[0][2] += imc * lam * rz;
+            v[h][0] -= imh * lam * rx; v[h][1] -= imh * lam * ry; v[h][2] -= imh * lam * rz;
+        }
+        converged = max_err <= tol;
+    }
+    for (int k = 0; k < g.n; ++k) g.v[k] = make_float3((float)v[k][0], (float)v[k][1], (float)v[k][2]);
+    return converged;
+}
+
+// shake_cluster_positions (nhs_amber_fused.cu:1889-2173), FP64 iteration:
+// corrections along the step-start bond, velocity impulse correction/dt.
+__device__ bool shake(GroupState &g, const float3 xs[4], const GroupParam &gp, float dt, double tol, int max_iter) {
+    if (g.n < 2) return true;
+    double v[4][3], x[4][3];
+    for (int k = 0; k < g.n; ++k) {
+        x[k][0] = g.x[k].x; x[k][1] = g.x[k].y; x[k][2] = g.x[k].z;
+        v[k][0] = g.v[k].x; v[k][1] = g.v[k].y; v[k][2] = g.v[k].z;
+    }
+    const double imc = gp.inv_mass_c, imh = gp.inv_mass_h, idt = 1.0 / (double)dt;
+    bool converged = false;
+    for (int it = 0; it < max_iter && !converged; ++it) {
+        double max_err = 0.0;
+        for (int h = 1; h < g.n; ++h) {
+            const double rx = x[h][0] - x[0][0], ry = x[h][1] - x[0][1], rz = x[h][2] - x[0][2];
+            const double ox = (double)xs[h].x - (double)xs[0].x, oy = (double)xs[h].y - (double)xs[0].y,
+                         oz = (double)xs[h].z - (double)xs[0].z;
+            const double d = gp.d[h - 1];
+            const double r2 = rx * rx + ry * ry + rz * rz, d2 = d * d, diff = r2 - d2;
+            if (!isfinite(r2) || !(d2 > 0.0)) return false;
+            const double rel = fabs(diff) / d2;
+            max_err = fmax(max_err, rel);
+            if (rel <= tol) continue;
+            const double rr0 = rx * ox + ry * oy + rz * oz;
+            const double scale = sqrt(r2 * (ox * ox + oy * oy + oz * oz));
+            if (!isfinite(rr0) || !(scale > 0.0) || fabs(rr0) / scale <= 1e-6) return false;
+            const double lam = diff / (2.0 * (imc + imh) * rr0);
+            x[0][0] += imc * lam * ox; x[0][1] += imc * lam * oy; x[0][2] += imc * lam * oz;
+            x[h][0] -= imh * lam * ox; x[h][1] -= imh * lam * oy; x[h][2] -= imh * lam * oz;
+            v[0][0] += imc * lam * ox * idt; v[0][1] += imc * lam * oy * idt; v[0][2] += imc * lam * oz * idt;
+            v[h][0] -= imh * lam * ox * idt; v[h][1] -= imh * lam * oy * idt; v[h][2] -= imh * lam * oz * idt;
+        }
+        converged = max_err <= tol;
+    }
+    for (int k = 0; k < g.n; ++k) {
+        g.x[k] = make_float3((float)x[k][0], (float)x[k][1], (float)x[k][2]);
+        g.v[k] = make_float3((float)v[k][0], (float)v[k][1], (float)v[k][2]);
+    }
+    return converged;
+}
+
+// langevin_thermostat with dt/2 (nhs_amber_fused.cu:1713-1744, :3656-3666).
+__device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float gamma, unsigned long long step,
+                               int purpose, const float *noise) {
+    const float c1 = expf(-gamma * (0.5f * D.dt));
+    for (int k = 0; k < g.n; ++k) {
+        float xi[3];
+        if (noise) {
+            xi[0] = noise[(size_t)g.a[k] * 3 + 0];
+            xi[1] = noise[(size_t)g.a[k] * 3 + 1];
+            xi[2] = noise[(size_t)g.a[k] * 3 + 2];
+        } else {
+            langevin_normals(D.seed_lo, D.stream, (uint32_t)(g.a[k] - D.atom_begin), step, purpose, xi);
+        }
+        const float c2 = sqrtf((1.0f - c1 * c1) * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T / g.m[k]);
+        g.v[k].x = c1 * g.v[k].x + c2 * xi[0];
+        g.v[k].y = c1 * g.v[k].y + c2 * xi[1];
+        g.v[k].z = c1 * g.v[k].z + c2 * xi[2];
+    }
+}
+
+// Half kick with the 1000 kcal/mol/A clamp of prism_asc_used_force
+// (prism_asc_decision.cuh:100-104) when gamma != 0, exact when gamma == 0
+// (nhs_amber_fused.cu:1750-1805, :3667-3703).
+__device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, float gamma) {
+    for (int k = 0; k < g.n; ++k) {
+        const long long *fa = P.f + (size_t)g.a[k] * 4;
+        float fx = from_fixed(fa[0], FORCE_FIXED_BITS), fy = from_fixed(fa[1], FORCE_FIXED_BITS),
+              fz = from_fixed(fa[2], FORCE_FIXED_BITS);
+        if (gamma != 0.0f) {
+            const float mag = sqrtf(fx * fx + fy * fy + fz * fz);
+            const float q = mag > MAX_FORCE ? MAX_FORCE / mag : 1.0f;
+            fx *= q; fy *= q; fz *= q;
+        }
+        const float kick = 0.5f * dt * (float)KCAL_TO_AMU_A2_PS2 / g.m[k];
+        g.v[k].x += kick * fx;
+        g.v[k].y += kick * fy;
+        g.v[k].z += kick * fz;
+    }
+}
+
+__device__ inline bool load_group(const DevPtrs &P, int gidx, GroupState &g, GroupParam &gp) {
+    const Group G = P.groups[gidx];
+    if (G.atom[0] < 0) return false;
+    gp = P.gparams[gidx];
+    g.n = 1 + gp.n_h;
+    for (int k = 0; k < 4; ++k) {
+        g.a[k] = G.atom[k];
+        if (k < g.n) {
+            g.m[k] = P.mass[G.atom[k]];
+            g.x[k] = f3(P.x[G.atom[k]]);
+            g.v[k] = f3(P.v[G.atom[k]]);
+        }
+    }
+    return true;
+}
+
+__device__ inline bool finite3(float3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+
+__global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *noise) {
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * IW + (threadIdx.x >> 6);
+    if (gw >= P.n_waves_arena) return;
+    const int s = P.wave_sys[gw];
+    if (s < 0) return;
+    const SysDesc &D = P.sys[s];
+    if (!D.active) return;
+    SysState &S = P.sst[s];
+    const unsigned long long step = S.step;
+    float T, gamma;
+    schedule(D, step + 1, T, gamma);
+    const bool com = D.com_every > 0 && step > 0 && step % (unsigned long long)D.com_every == 0;
+    GroupState g;
+    GroupParam gp;
+    bool ok = true, moved = false;
+    if (load_group(P, gw * 64 + lane, g, gp)) {
+        if (com)
+            for (int k = 0; k < g.n; ++k) {
+                g.v[k].x -= (float)S.com_v[0];
+                g.v[k].y -= (float)S.com_v[1];
+                g.v[k].z -= (float)S.com_v[2];
+            }
+        if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 1, noise);
+        ok &= rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+        half_kick(g, P, D.dt, gamma);
+        ok &= rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+        float3 xs[4];
+        for (int k = 0; k < g.n; ++k) {
+            xs[k] = g.x[k];
+            g.x[k].x += D.dt * g.v[k].x;
+            g.x[k].y += D.dt * g.v[k].y;
+            g.x[k].z += D.dt * g.v[k].z;
+        }
+        ok &= shake(g, xs, gp, D.dt, D.shake_tol, D.constraint_max_iter);
+        for (int k = 0; k < g.n; ++k) {
+            const int a = g.a[k];
+            ok &= finite3(g.x[k]) && finite3(g.v[k]);
+            P.x[a] = make_float4(g.x[k].x, g.x[k].y, g.x[k].z, 0.0f);
+            P.v[a] = make_float4(g.v[k].x, g.v[k].y, g.v[k].z, 0.0f);
+            P.xs[a] = make_float4(xs[k].x, xs[k].y, xs[k].z, 0.0f);
+            long long *fa = P.f + (size_t)a * 4;
+            fa[0] = 0; fa[1] = 0; fa[2] = 0;
+            *(float3 *)&P.xq[P.slot_of[a]] = g.x[k];
+            const float4 b = P.x_build[a];
+            const float dx = g.x[k].x - b.x, dy = g.x[k].y - b.y, dz = g.x[k].z - b.z;
+            moved |= dx * dx + dy * dy + dz * dz > D.half_skin2;
+        }
+    }
+    const bool due = (long long)step - S.rebuild_step >= (long long)D.rebuild_every;
+    if (__ballot(moved) != 0ull || due) {
+        if (lane == 0) S.rebuild_flag = 1;
+    }
+    if (__ballot(!ok) != 0ull && lane == 0) {
+        atomicOr(&S.status, ST_CONSTRAINT);
+    }
+}
+
+__global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float *noise) {
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * IW + (threadIdx.x >> 6);
+    __shared__ int last;
+    const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
+    if (s >= 0 && P.sys[s].active) {
+        const SysDesc &D = P.sys[s];
+        SysState &S = P.sst[s];
+        const unsigned long long step = S.step;
+        float T, gamma;
+        schedule(D, step + 1, T, gamma);
+        const bool next_com = D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0;
+        GroupState g;
+        GroupParam gp;
+        bool ok = true;
+        double mv[3] = {0.0, 0.0, 0.0};
+        if (load_group(P, gw * 64 + lane, g, gp)) {
+            half_kick(g, P, D.dt, gamma);
+            ok &= rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 2, noise);
+            for (int k = 0; k < g.n; ++k) {  // nhs_velocity_safety_clamp (nhs_amber_fused.cu:11148-11186)
+                const float thermal = sqrtf((float)KCAL_TO_AMU_A2_PS2 * (float)KB * T / fmaxf(g.m[k], 1.0f));
+                const float vmax = fmaxf(20.0f * thermal, 2.0f);
+                const float sp = sqrtf(g.v[k].x * g.v[k].x + g.v[k].y * g.v[k].y + g.v[k].z * g.v[k].z);
+                if (sp > vmax) {
+                    const float sc = vmax / sp;
+                    g.v[k].x *= sc; g.v[k].y *= sc; g.v[k].z *= sc;
+                }
+            }
+            ok &= rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            for (int k = 0; k < g.n; ++k) {
+                ok &= finite3(g.v[k]);
+                P.v[g.a[k]] = make_float4(g.v[k].x, g.v[k].y, g.v[k].z, 0.0f);
+                mv[0] += (double)g.m[k] * g.v[k].x;
+                mv[1] += (double)g.m[k] * g.v[k].y;
+                mv[2] += (double)g.m[k] * g.v[k].z;
+            }
+        }
+        if (next_com) {
+            for (int c = 0; c < 3; ++c) mv[c] = wave_sum(mv[c]);
+            if (lane == 0)
+                for (int c = 0; c < 3; ++c) P.com_partial[(size_t)gw * 3 + c] = mv[c];
+        }
+        if (__ballot(!ok) != 0ull && lane == 0) atomicOr(&S.status, ST_CONSTRAINT);
+    }
+    // Last-block handshake (no waiting): the workgroup that retires last sums
+    // the COM partials in wave order and advances every system's step counter.
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&P.counters->done_blocks, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int q = threadIdx.x; q < P.n_sys; q += blockDim.x) {
+        const SysDesc &D = P.sys[q];
+        if (!D.active) continue;
+        SysState &S = P.sst[q];
+        const unsigned long long step = S.step;
+        if (D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0) {
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (int w = D.wave_begin; w < D.wave_begin + D.n_waves; ++w)
+                for (int c = 0; c < 3; ++c) acc[c] += P.com_partial[(size_t)w * 3 + c];
+            for (int c = 0; c < 3; ++c) S.com_v[c] = acc[c] * D.inv_total_mass;
+        }
+        S.step = step + 1;
+    }
+    if (threadIdx.x == 0) P.counters->done_blocks = 0;
+}
+
+// Per-system reductions in FP64 (one workgroup per system, fixed order):
+// kinetic energy and centre-of-mass velocity.
+__global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs P, double *out /*[S][4]*/) {
+    const int s = blockIdx.x;
+    if (s >= P.n_sys) return;
+    const SysDesc &D = P.sys[s];
+    __shared__ double red[4][4];
+    double ke = 0.0, p[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < D.n_atoms; i += blockDim.x) {
+        const int a = D.atom_begin + i;
+        const float4 v = P.v[a];
+        const double m = P.mass[a];
+        ke += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+        p[0] += m * v.x; p[1] += m * v.y; p[2] += m * v.z;
+    }
+    ke = wave_sum(ke);
+    for (int c = 0; c < 3; ++c) p[c] = wave_sum(p[c]);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[w][0] = ke; red[w][1] = p[0]; red[w][2] = p[1]; red[w][3] = p[2]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k)
+            for (int c = 0; c < 4; ++c) r[c] += red[k][c];
+        out[s * 4 + 0] = r[0] / KCAL_TO_AMU_A2_PS2;
+        for (int c = 0; c < 3; ++c) {
+            out[s * 4 + 1 + c] = r[1 + c] * D.inv_total_mass;
+            P.sst[s].com_v[c] = r[1 + c] * D.inv_total_mass;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t st) {
+    const int grid = (P.n_waves_arena + IW - 1) / IW;
+    hipLaunchKernelGGL(k_first_half, dim3(grid), dim3(64 * IW), 0, st, P, noise);
+}
+
+void launch_second_half(const DevPtrs &P, const float *noise, hipStream_t st) {
+    const int grid = (P.n_waves_arena + IW - 1) / IW;
+    hipLaunchKernelGGL(k_second_half, dim3(grid), dim3(64 * IW), 0, st, P, noise);
+}
+
+void launch_kinetic(const DevPtrs &P, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sys_reduce, dim3(P.n_sys), dim3(256), 0, st, P, out);
+}
+
+}  // namespace prism

@@ -1,0 +1,109 @@
+// md_layout.hpp -- device data layout shared by the host engine and the HIP
+// kernels of the classical MD step.
+//
+// One engine holds S >= 1 independent systems packed into shared arenas:
+//   atoms     [atom_begin, atom_begin + n_atoms) per system; atom_begin is a
+//             multiple of BLOCK so the spatially sorted slots of a system
+//             occupy whole 64-slot blocks [block_begin, block_begin + n_blocks);
+//   groups    constraint groups (one heavy atom with its hydrogens, or a lone
+//             atom), padded per system to whole 64-lane waves;
+//   terms     bonds / angles / dihedrals / exceptions with arena atom indices;
+//   tiles     per i-block a row of up to MAX_ROW j-blocks (the cluster-pair
+//             list), with per-row exclusion bit masks.
+// Every kernel is launched over a fixed arena extent and skips padding, so a
+// captured graph stays valid while systems come and go.
+#pragma once
+#include <stdint.h>
+
+namespace prism {
+
+constexpr int BLOCK = 64;            // atoms per spatial block (= wavefront)
+constexpr int MAX_ROW = 512;         // j-blocks per i-block row (full list)
+constexpr int MAX_MASKS = 32;        // exclusion-mask tiles per row
+constexpr int SUB = 16;              // i-atoms per nonbonded wave (block quarter)
+constexpr int QUARTERS = BLOCK / SUB;
+constexpr int CHUNK = 8;             // tiles per nonbonded work unit
+constexpr int CHUNKS_PER_ROW = MAX_ROW / CHUNK;
+constexpr int UNITS_PER_BLOCK = QUARTERS * CHUNKS_PER_ROW;
+constexpr int FORCE_FIXED_BITS = 24; // int64 fixed point: 2^24 per kcal/mol/A
+constexpr int ENERGY_FIXED_BITS = 24;
+constexpr float DUMMY_FAR = 1.0e7f;  // padding slots sit far from every atom
+
+constexpr double COULOMB = 332.0636;         // nhs_amber_fused.cu:66
+constexpr double KCAL_TO_AMU_A2_PS2 = 418.4; // nhs_amber_fused.cu:69
+constexpr double KB = 0.001987204;           // nhs_amber_fused.cu:70
+constexpr float MAX_FORCE = 1000.0f;         // nhs_amber_fused.cu:71
+
+// status bits latched on the device (SysState.status)
+enum : int {
+    ST_CONSTRAINT = 1,   // SHAKE/RATTLE did not converge
+    ST_NONFINITE = 2,    // NaN/Inf in state or forces
+    ST_ROW_OVERFLOW = 4, // more than MAX_ROW tiles in a row
+    ST_MASK_OVERFLOW = 8,
+    ST_EXCEPTION_COINCIDENT = 16,
+};
+
+// Immutable per-system description (host-written).
+struct SysDesc {
+    int atom_begin, n_atoms;
+    int block_begin, n_blocks;
+    int wave_begin, n_waves;        // constraint-group waves
+    int active;
+    int rebuild_every;
+    int com_every;
+    int constraint_max_iter;
+    int sort_bits;                  // max bits per axis of the Hilbert sort key
+    int pad0;
+    float dt;
+    float cutoff, cutoff2, list_cutoff;
+    float half_skin2;               // (skin/2)^2 displacement trigger
+    float shake_tol_f, rattle_tol_f; // unused (FP64 copies below)
+    double shake_tol, rattle_tol;
+    double inv_total_mass;
+    // schedule (protocol_director.cu:58-123)
+    float t_start, t_end, gamma_base, gamma_eq;
+    int cold_hold_end, ramp_end, warm_hold_end, ramp_down_end;
+    int warm_tail, equilibration_steps;
+    uint32_t seed_lo, stream;
+    int noise_explicit;             // 1: read normals from the explicit array
+    int pad1;
+};
+
+// Mutable per-system state (device-written).
+struct SysState {
+    unsigned long long step;        // index of the step being executed
+    long long rebuild_step;         // step whose positions the pair list was built at (-1: before step 0)
+    int rebuild_flag;               // set by the integrator or the host, consumed by the sort
+    int rows_dirty;                 // set by the sort for the row builder of the same rebuild
+    int status;                     // ST_* bits
+    int n_rebuilds;
+    double com_v[3];                // centre-of-mass velocity for the next COM step
+    long long energy[8];            // fixed-point term energies (5 used)
+    unsigned long long force_evals;
+};
+
+// Global engine counters (device-written).
+struct EngineCounters {
+    unsigned int done_blocks;       // last-block handshake of the integrator
+    unsigned int rows_done;         // last-block handshake of the row builder
+    int any_dirty;                  // some system rebuilt its rows this step
+    int n_units;                    // nonbonded work units in the unit list
+};
+
+struct Group {                      // constraint group
+    int atom[4];                    // central, h0, h1, h2 (-1 unused); atom[0] < 0 => padding
+};
+
+struct GroupParam {
+    float d[3];                     // X-H bond lengths
+    float inv_mass_c, inv_mass_h;
+    int n_h;
+    int pad[2];
+};
+
+struct Bond { int i, j; float r0, k; };
+struct Angle { int i, j, k; float theta0, kf; int pad[3]; };
+struct Dihedral { int i, j, k, l; int n; float phase, kf; int pad; };
+struct Exception14 { int i, j; float qq, sigma, eps; int pad[3]; };
+
+}  // namespace prism
