@@ -166,6 +166,8 @@ typedef struct PrismMdStatsV1 {
     int32_t  n_tiles;             /* cluster-pair tiles in the current list           */
     int32_t  n_blocks;            /* 64-atom spatial blocks                           */
     int32_t  status;              /* latched device status bits (0 = clean)           */
+    int32_t  max_shake_sweeps;    /* most SHAKE sweeps any cluster needed so far       */
+    int32_t  max_rattle_sweeps;   /* most RATTLE sweeps any cluster needed so far      */
     int32_t  reserved;
 } PrismMdStatsV1;
 

@@ -274,6 +274,7 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
         hgp.insert(hgp.end(), gpp.begin(), gpp.end());
         for (int w = 0; w < nw; ++w) hwave_sys.push_back(s);
         const int o = H.atom_begin;
+        const int tb0 = (int)hb.size(), ta0 = (int)ha.size(), td0 = (int)hd.size(), te0 = (int)hx14.size();
         for (int b = 0; b < t->n_bonds; ++b)
             hb.push_back(Bond{o + t->bond_atoms[2 * b], o + t->bond_atoms[2 * b + 1], t->bond_r0[b], t->bond_k[b]});
         for (int a = 0; a < t->n_angles; ++a)
@@ -337,10 +338,44 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
         D.equilibration_steps = p.equilibration_steps;
         D.seed_lo = (uint32_t)p.seed;
         D.stream = p.stream_id;
+        D.bond_begin = tb0;
+        D.n_bonds = t->n_bonds;
+        D.angle_begin = ta0;
+        D.n_angles = t->n_angles;
+        D.dih_begin = td0;
+        D.n_dih = t->n_dihedrals;
+        D.exc_begin = te0;
+        D.n_exc = t->n_exceptions;
         desc_.push_back(D);
         sys_.push_back(std::move(H));
     }
     const int B = A / BLOCK;
+    // ---- bonded contribution slots, atom-major: the slots of one atom are
+    // contiguous and ordered by term kind, term index and position, so the
+    // gather adds every atom's bonded forces in one fixed order
+    std::vector<int> inc_off(A + 1, 0);
+    auto count_atoms = [&](const int *atoms, size_t n_terms, int arity, size_t stride) {
+        for (size_t k = 0; k < n_terms; ++k)
+            for (int p = 0; p < arity; ++p) ++inc_off[atoms[k * stride + p] + 1];
+    };
+    count_atoms(hb.empty() ? nullptr : &hb[0].i, hb.size(), 2, sizeof(Bond) / sizeof(int));
+    count_atoms(ha.empty() ? nullptr : &ha[0].i, ha.size(), 3, sizeof(Angle) / sizeof(int));
+    count_atoms(hd.empty() ? nullptr : &hd[0].i, hd.size(), 4, sizeof(Dihedral) / sizeof(int));
+    count_atoms(hx14.empty() ? nullptr : &hx14[0].i, hx14.size(), 2, sizeof(Exception14) / sizeof(int));
+    for (int a = 0; a < A; ++a) inc_off[a + 1] += inc_off[a];
+    std::vector<int> fill(inc_off.begin(), inc_off.end() - 1);
+    auto assign = [&](const int *atoms, size_t n_terms, int arity, size_t stride, std::vector<int> &dst) {
+        dst.resize(n_terms * arity);
+        for (size_t k = 0; k < n_terms; ++k)
+            for (int p = 0; p < arity; ++p) dst[k * arity + p] = fill[atoms[k * stride + p]]++;
+    };
+    std::vector<int> dst_b, dst_a, dst_d, dst_e;
+    assign(hb.empty() ? nullptr : &hb[0].i, hb.size(), 2, sizeof(Bond) / sizeof(int), dst_b);
+    assign(ha.empty() ? nullptr : &ha[0].i, ha.size(), 3, sizeof(Angle) / sizeof(int), dst_a);
+    assign(hd.empty() ? nullptr : &hd[0].i, hd.size(), 4, sizeof(Dihedral) / sizeof(int), dst_d);
+    assign(hx14.empty() ? nullptr : &hx14[0].i, hx14.size(), 2, sizeof(Exception14) / sizeof(int), dst_e);
+    const size_t n_slots = (size_t)inc_off[A];
+    const size_t n_terms = hb.size() + ha.size() + hd.size() + hx14.size();
     // ---- device allocation
     P_.n_sys = (int)systems.size();
     P_.n_atoms_arena = A;
@@ -366,6 +401,14 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
     P_.masks = dalloc<unsigned long long>((size_t)B * MAX_MASKS * 64);
     P_.unit_count = dalloc<int>(B);
     P_.units = dalloc<int>((size_t)B * UNITS_PER_BLOCK);
+    P_.unit_off = dalloc<int>(B);
+    P_.unit_f = dalloc<float4>((size_t)B * UNITS_PER_BLOCK * SUB);
+    P_.contrib = dalloc<float>(n_slots * 3);
+    int *d_inc_off = dalloc<int>(inc_off.size());
+    int *d_dst_b = dalloc<int>(dst_b.size()), *d_dst_a = dalloc<int>(dst_a.size());
+    int *d_dst_d = dalloc<int>(dst_d.size()), *d_dst_e = dalloc<int>(dst_e.size());
+    P_.term_e = dalloc<double>(n_terms);
+    P_.atom_e = dalloc<double>(A);
     int *d_skip_off = dalloc<int>(skip_off.size()), *d_skip_idx = dalloc<int>(skip_idx.size());
     int *d_ex_off = dalloc<int>(ex_off.size()), *d_ex_idx = dalloc<int>(ex_idx.size());
     Group *d_groups = dalloc<Group>((size_t)W * 64);
@@ -394,6 +437,8 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
     P_.skip_off = d_skip_off; P_.skip_idx = d_skip_idx; P_.ex_off = d_ex_off; P_.ex_idx = d_ex_idx;
     P_.groups = d_groups; P_.gparams = d_gp; P_.wave_sys = d_wave_sys;
     P_.bonds = d_b; P_.angles = d_a; P_.dih = d_d; P_.exc = d_e;
+    P_.inc_off = d_inc_off;
+    P_.dst_bond = d_dst_b; P_.dst_angle = d_dst_a; P_.dst_dih = d_dst_d; P_.dst_exc = d_dst_e;
     P_.n_bonds = (int)hb.size(); P_.n_angles = (int)ha.size(); P_.n_dih = (int)hd.size(); P_.n_exc = (int)hx14.size();
     P_.sys = d_desc_;
     int rc = PRISM_MD_OK;
@@ -402,12 +447,15 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
         (rc = upload(d_block_sys, hblock_sys)) || (rc = upload(d_skip_off, skip_off)) || (rc = upload(d_skip_idx, skip_idx)) ||
         (rc = upload(d_ex_off, ex_off)) || (rc = upload(d_ex_idx, ex_idx)) || (rc = upload(d_groups, hgroups)) ||
         (rc = upload(d_gp, hgp)) || (rc = upload(d_wave_sys, hwave_sys)) || (rc = upload(d_b, hb)) || (rc = upload(d_a, ha)) ||
-        (rc = upload(d_d, hd)) || (rc = upload(d_e, hx14)) || (rc = upload(d_desc_, desc_)))
+        (rc = upload(d_d, hd)) || (rc = upload(d_e, hx14)) || (rc = upload(d_desc_, desc_)) ||
+        (rc = upload(d_inc_off, inc_off)) || (rc = upload(d_dst_b, dst_b)) || (rc = upload(d_dst_a, dst_a)) ||
+        (rc = upload(d_dst_d, dst_d)) || (rc = upload(d_dst_e, dst_e)))
         return rc;
     std::vector<SysState> st(desc_.size());
     for (auto &S : st) {
         std::memset(&S, 0, sizeof(S));
         S.rebuild_step = -1;
+        S.fault_atom = -1;
     }
     if ((rc = upload(P_.sst, st))) return rc;
     launch_kinetic(P_, d_red_, stream_);  // initial COM velocity (zero by construction)
@@ -431,9 +479,9 @@ int Engine::refresh_all() {
 }
 
 void Engine::full_force() {
-    launch_zero_forces(P_, stream_);
     launch_nonbonded(P_, false, stream_);
     launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
+    launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
     ++force_evals_;
 }
 
@@ -443,6 +491,7 @@ void Engine::launch_step(const float *noise_n, const float *noise_np1) {
     launch_rows(P_, stream_);
     launch_nonbonded(P_, false, stream_);
     launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
+    launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
     launch_second_half(P_, noise_np1, stream_);
     ++force_evals_;
 }
@@ -454,9 +503,14 @@ int Engine::check_status() {
     for (size_t s = 0; s < st.size(); ++s) {
         const int b = st[s].status;
         if (!b) continue;
-        const std::string where = " (system " + std::to_string(s) + ", step " + std::to_string(st[s].step) + ")";
-        if (b & ST_CONSTRAINT) return fail(PRISM_MD_E_CONSTRAINT, "SHAKE/RATTLE did not converge or state non-finite" + where);
+        static const char *kinds[] = {"?", "RATTLE after OU_N", "RATTLE before drift", "SHAKE", "RATTLE before OU_NP1",
+                                      "final RATTLE", "non-finite state"};
+        const int fk = st[s].fault_kind >= 0 && st[s].fault_kind <= 6 ? st[s].fault_kind : 0;
+        const std::string where = " (system " + std::to_string(s) + ", " + kinds[fk] + ", atom " +
+                                  std::to_string(st[s].fault_atom - sys_[s].atom_begin) + ", step " +
+                                  std::to_string(st[s].fault_step) + ")";
         if (b & ST_NONFINITE) return fail(PRISM_MD_E_NONFINITE, "non-finite state" + where);
+        if (b & ST_CONSTRAINT) return fail(PRISM_MD_E_CONSTRAINT, "constraint iteration did not converge" + where);
         if (b & ST_EXCEPTION_COINCIDENT) return fail(PRISM_MD_E_NONFINITE, "coincident exception atoms" + where);
         if (b & ST_ROW_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "tile row overflow" + where);
         if (b & ST_MASK_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "exclusion-mask overflow" + where);
@@ -533,11 +587,11 @@ int Engine::get_state(int s, float *x, float *v, int flags) {
 int Engine::compute_forces(unsigned mask, int s, float *F_out, int64_t *F_fixed, double *E_terms) {
     if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
     mask &= PRISM_MD_TERM_ALL;
-    for (int q = 0; q < (int)sys_.size(); ++q)
-        if (int rc = hipcheck(hipMemsetAsync(&P_.sst[q].energy[0], 0, sizeof(long long) * 8, stream_), "energy zero")) return rc;
-    launch_zero_forces(P_, stream_);
-    if (mask & PRISM_MD_TERM_NONBONDED) launch_nonbonded(P_, E_terms != nullptr, stream_);
-    launch_bonded(P_, mask, E_terms != nullptr, stream_);
+    const bool energy = E_terms != nullptr;
+    if (mask & PRISM_MD_TERM_NONBONDED) launch_nonbonded(P_, energy, stream_);
+    launch_bonded(P_, mask, energy, stream_);
+    launch_gather(P_, mask, energy, stream_);
+    if (energy) launch_energy_reduce(P_, stream_);
     if (int rc = hipcheck(hipGetLastError(), "force launch")) return rc;
     if (int rc = check_status()) return rc;
     const HostSys &H = sys_[s];
@@ -553,7 +607,7 @@ int Engine::compute_forces(unsigned mask, int s, float *F_out, int64_t *F_fixed,
     if (E_terms) {
         SysState S;
         if (int rc = d2h(&S, P_.sst + s, sizeof(S))) return rc;
-        for (int k = 0; k < PRISM_MD_N_TERMS; ++k) E_terms[k] = (double)S.energy[k] / (double)(1ll << ENERGY_FIXED_BITS);
+        for (int k = 0; k < PRISM_MD_N_TERMS; ++k) E_terms[k] = S.energy[k];
     }
     if (mask != PRISM_MD_TERM_ALL) full_force();  // restore the carried F(x)
     return check_status();
@@ -778,6 +832,8 @@ int Engine::stats(int s, PrismMdStatsV1 *out) {
     out->n_tiles = std::accumulate(rn.begin(), rn.end(), 0);
     out->n_blocks = H.n_blocks;
     out->status = S.status;
+    out->max_shake_sweeps = S.max_shake_iter;
+    out->max_rattle_sweeps = S.max_rattle_iter;
     return PRISM_MD_OK;
 }
 

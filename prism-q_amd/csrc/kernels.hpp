@@ -23,7 +23,16 @@ struct DevPtrs {
     const int *block_sys;           // system of each block (-1 padding)
     int *row_n, *row_j, *row_m;     // [B], [B][MAX_ROW], [B][MAX_ROW]
     int *unit_count;                // [B] nonbonded work units of each block's row
+    int *unit_off;                  // [B] first unit of each block in the unit list
     int *units;                     // [B * UNITS_PER_BLOCK] (block << 8 | quarter << 6 | chunk)
+    float4 *unit_f;                 // [B * UNITS_PER_BLOCK][SUB] per-unit i-forces (x, y, z, energy)
+    // bonded contributions: slot per (term, atom position), atom-major so the
+    // slots of one atom are contiguous [inc_off[a], inc_off[a+1])
+    float *contrib;                 // [n_slots][3]
+    const int *inc_off;             // [A + 1]
+    const int *dst_bond, *dst_angle, *dst_dih, *dst_exc;  // first slot per term position: [n][arity]
+    double *term_e;                 // per-term energy (bonds, angles, dihedrals, exceptions)
+    double *atom_e;                 // per-atom nonbonded energy of the last gather
     unsigned long long *masks;      // [B][MAX_MASKS][64]
     // partner lists over arena atoms
     const int *skip_off, *skip_idx; // exclusions U exception partners (nonbonded skip)
@@ -64,6 +73,11 @@ void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);
 void launch_second_half(const DevPtrs &P, const float *noise, hipStream_t s);
 void launch_zero_forces(const DevPtrs &P, hipStream_t s);
+// Sum the nonbonded unit partials and the bonded slots of every atom, in fixed
+// order, into the int64 fixed-point accumulator (one writer per atom).
+void launch_gather(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
+// Per-system FP64 energy sums of the last evaluation into SysState.energy.
+void launch_energy_reduce(const DevPtrs &P, hipStream_t s);
 void launch_kinetic(const DevPtrs &P, double *ke_out, hipStream_t s);
 void launch_emit_pairs(const DevPtrs &P, int mode, RefGrid g, int *out_i, int *out_j, unsigned long long *count,
                        unsigned long long cap, hipStream_t s);

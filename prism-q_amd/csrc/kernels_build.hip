@@ -319,6 +319,9 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
     __shared__ int row_lds[ROW_WAVES][MAX_ROW];
     __shared__ int flag_lds[ROW_WAVES][MAX_ROW];
     __shared__ int last;
+    // the sort of this step raised any_dirty when it re-sorted some system;
+    // without it no row changes and the unit list stays valid
+    if (P.counters->any_dirty == 0) return;
     if (b < P.n_blocks_arena) {
         const int s = P.block_sys[b];
         if (s >= 0 && P.sys[s].active && P.sst[s].rows_dirty) build_row(P, b, wave, lane, row_lds, flag_lds);
@@ -332,13 +335,10 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
     __syncthreads();
     if (!last) return;
     __threadfence();
-    __shared__ int dirty, base;
-    if (threadIdx.x == 0) {
-        dirty = P.counters->any_dirty;
-        base = 0;
-    }
+    __shared__ int base;
+    if (threadIdx.x == 0) base = 0;
     __syncthreads();
-    if (dirty) {
+    {
         for (int b0 = 0; b0 < P.n_blocks_arena; b0 += 64 * ROW_WAVES) {
             const int bb = b0 + threadIdx.x;
             int cnt = 0;
@@ -357,6 +357,7 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
                 __syncthreads();
             }
             const int off = base + scan[threadIdx.x] - cnt;
+            if (bb < P.n_blocks_arena) P.unit_off[bb] = off;
             const int per_q = cnt / QUARTERS;  // chunks per quarter
             for (int k = 0; k < cnt; ++k) {
                 const int q = k / per_q, c = k % per_q;

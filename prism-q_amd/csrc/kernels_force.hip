@@ -139,20 +139,16 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
             }
             wave_lds_fence();
         }
-        // sum the four j-slot partials of each i-atom (lanes il, il+16, il+32, il+48)
+        // sum the four j-slot partials of each i-atom (lanes il, il+16, il+32,
+        // il+48) in a fixed order and store the unit's partial; the gather
+        // adds the partials of an atom's units in unit order
         fx += __shfl_xor(fx, 16); fy += __shfl_xor(fy, 16); fz += __shfl_xor(fz, 16);
         fx += __shfl_xor(fx, 32); fy += __shfl_xor(fy, 32); fz += __shfl_xor(fz, 32);
-        if (js == 0 && ai >= 0) {
-            add_fixed(&P.f[(size_t)ai * 4 + 0], fx, FORCE_FIXED_BITS);
-            add_fixed(&P.f[(size_t)ai * 4 + 1], fy, FORCE_FIXED_BITS);
-            add_fixed(&P.f[(size_t)ai * 4 + 2], fz, FORCE_FIXED_BITS);
-        }
         if (ENERGY) {
-            const double es = wave_sum((double)e);
-            if (lane == 0)
-                atomicAdd((unsigned long long *)&P.sst[s].energy[3],
-                          (unsigned long long)(long long)rint(es * (double)(1ll << ENERGY_FIXED_BITS)));
+            e += __shfl_xor(e, 16);
+            e += __shfl_xor(e, 32);
         }
+        if (js == 0) P.unit_f[(size_t)u * SUB + il] = make_float4(fx, fy, fz, ENERGY ? e : 0.0f);
     }
 }
 
@@ -164,122 +160,198 @@ __device__ inline double3 ld3d(const float4 *x, int a) {
     const float4 p = x[a];
     return make_double3(p.x, p.y, p.z);
 }
-__device__ inline void add_fixed_d(long long *acc, double v) {
-    double d = v * (double)(1ll << FORCE_FIXED_BITS);
-    d = fmin(fmax(d, -4.0e18), 4.0e18);
-    atomicAdd((unsigned long long *)acc, (unsigned long long)(long long)rint(d));
-}
-__device__ inline void add_force(const DevPtrs &P, int a, double x, double y, double z) {
-    add_fixed_d(&P.f[(size_t)a * 4 + 0], x);
-    add_fixed_d(&P.f[(size_t)a * 4 + 1], y);
-    add_fixed_d(&P.f[(size_t)a * 4 + 2], z);
-}
-__device__ inline void add_energy(const DevPtrs &P, int atom, int term, double v) {
-    const int s = P.atom_sys[atom];
-    atomicAdd((unsigned long long *)&P.sst[s].energy[term],
-              (unsigned long long)(long long)rint(v * (double)(1ll << ENERGY_FIXED_BITS)));
-}
 __device__ inline double3 sub3(double3 a, double3 b) { return make_double3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ inline double dot3(double3 a, double3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline double3 cross3(double3 a, double3 b) {
     return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+__device__ inline void put(const DevPtrs &P, int slot, double x, double y, double z) {
+    float *c = P.contrib + (size_t)slot * 3;
+    c[0] = (float)x;
+    c[1] = (float)y;
+    c[2] = (float)z;
+}
 
 // One thread per term: bonds, angles, dihedrals, exceptions in that order.
-// The bonded terms and exceptions evaluate in FP64 from the FP32 state: on
-// CDNA4 the FP64 vector rate equals the unpacked FP32 rate, and it removes the
+// Every term writes the force on each of its atoms into that atom's
+// contribution slot (zeros when the term is masked out or degenerate); the
+// gather adds an atom's slots in slot order, so there are no atomics.  The
+// bonded terms and exceptions evaluate in FP64 from the FP32 state: on CDNA4
+// the FP64 vector rate equals the unpacked FP32 rate, and it removes the
 // cancellation of r - r0 and theta - theta0 near equilibrium that limits an
 // FP32 evaluation to ~1e-5 relative.
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int idx = tid;
+    double energy = 0.0;
     if (idx < P.n_bonds) {
-        if (!(mask & PRISM_MD_TERM_BOND)) return;
         const Bond B = P.bonds[idx];
-        if (B.i < 0) return;
-        const double3 r = sub3(ld3d(P.x, B.j), ld3d(P.x, B.i));
-        const double len = sqrt(dot3(r, r));
-        if (len < 1e-8) return;
-        const double dr = len - (double)B.r0;
-        const double fm = -2.0 * (double)B.k * dr / len;
-        add_force(P, B.i, -fm * r.x, -fm * r.y, -fm * r.z);
-        add_force(P, B.j, fm * r.x, fm * r.y, fm * r.z);
-        if (ENERGY) add_energy(P, B.i, 0, (double)B.k * dr * dr);
-        return;
-    }
-    idx -= P.n_bonds;
-    if (idx < P.n_angles) {
-        if (!(mask & PRISM_MD_TERM_ANGLE)) return;
-        const Angle A = P.angles[idx];
-        if (A.i < 0) return;
-        const double3 pj = ld3d(P.x, A.j);
-        const double3 a = sub3(ld3d(P.x, A.i), pj), b = sub3(ld3d(P.x, A.k), pj);
-        const double la = sqrt(dot3(a, a)), lb = sqrt(dot3(b, b));
-        if (la < 1e-8 || lb < 1e-8) return;
-        double cs = dot3(a, b) / (la * lb);
-        cs = fmax(-1.0, fmin(1.0, cs));
-        const double dth = acos(cs) - (double)A.theta0;
-        double sn = sqrt(1.0 - cs * cs);
-        if (sn < 1e-8) sn = 1e-8;
-        const double fm = -2.0 * (double)A.kf * dth / sn;
-        const double ia = 1.0 / la, ib = 1.0 / lb;
-        const double3 di = make_double3((b.x * ib - cs * a.x * ia) * ia, (b.y * ib - cs * a.y * ia) * ia, (b.z * ib - cs * a.z * ia) * ia);
-        const double3 dk = make_double3((a.x * ia - cs * b.x * ib) * ib, (a.y * ia - cs * b.y * ib) * ib, (a.z * ia - cs * b.z * ib) * ib);
-        add_force(P, A.i, -fm * di.x, -fm * di.y, -fm * di.z);
-        add_force(P, A.k, -fm * dk.x, -fm * dk.y, -fm * dk.z);
-        add_force(P, A.j, fm * (di.x + dk.x), fm * (di.y + dk.y), fm * (di.z + dk.z));
-        if (ENERGY) add_energy(P, A.i, 1, (double)A.kf * dth * dth);
-        return;
-    }
-    idx -= P.n_angles;
-    if (idx < P.n_dih) {
-        if (!(mask & PRISM_MD_TERM_DIHEDRAL)) return;
-        const Dihedral Q = P.dih[idx];
-        if (Q.i < 0) return;
-        const double3 pj = ld3d(P.x, Q.j), pk = ld3d(P.x, Q.k);
-        const double3 v0 = sub3(ld3d(P.x, Q.i), pj), v1 = sub3(pk, pj), v2 = sub3(pk, ld3d(P.x, Q.l));
-        const double3 c0 = cross3(v0, v1), c1 = cross3(v1, v2);
-        const double n1 = dot3(c0, c0), n2 = dot3(c1, c1), nb = dot3(v1, v1);
-        if (n1 < 1e-16 || n2 < 1e-16 || nb < 1e-16) return;
-        const double lb = sqrt(nb);
-        // phi from atan2 of |c0 x c1| and c0.c1: the reference's acos / asin
-        // branches (nhs_amber_fused.cu:562-580) in exact arithmetic
-        const double3 cc = cross3(c0, c1);
-        double phi = atan2(sqrt(dot3(cc, cc)), dot3(c0, c1));
-        if (dot3(v0, c1) < 0.0) phi = -phi;
-        const double nn = (double)Q.n;
-        const double dE = -(double)Q.kf * nn * sin(nn * phi - (double)Q.phase);
-        const double ff0 = (-dE * lb) / n1, ff3 = (dE * lb) / n2;
-        const double ff1 = dot3(v0, v1) / nb, ff2 = dot3(v2, v1) / nb;
-        const double3 f1 = make_double3(ff0 * c0.x, ff0 * c0.y, ff0 * c0.z);
-        const double3 f4 = make_double3(ff3 * c1.x, ff3 * c1.y, ff3 * c1.z);
-        const double3 sv = make_double3(ff1 * f1.x - ff2 * f4.x, ff1 * f1.y - ff2 * f4.y, ff1 * f1.z - ff2 * f4.z);
-        add_force(P, Q.i, f1.x, f1.y, f1.z);
-        add_force(P, Q.j, sv.x - f1.x, sv.y - f1.y, sv.z - f1.z);
-        add_force(P, Q.k, -sv.x - f4.x, -sv.y - f4.y, -sv.z - f4.z);
-        add_force(P, Q.l, f4.x, f4.y, f4.z);
-        if (ENERGY) add_energy(P, Q.i, 2, (double)Q.kf * (1.0 + cos(nn * phi - (double)Q.phase)));
-        return;
-    }
-    idx -= P.n_dih;
-    if (idx < P.n_exc) {
-        if (!(mask & PRISM_MD_TERM_EXCEPTION)) return;
-        const Exception14 X = P.exc[idx];
-        if (X.i < 0) return;
-        const double3 d = sub3(ld3d(P.x, X.i), ld3d(P.x, X.j));
-        const double r2 = dot3(d, d);
-        if (r2 < 1e-8) {  // coincident exception atoms (nhs_amber_fused.cu:3338-3343)
-            atomicOr(&P.sst[P.atom_sys[X.i]].status, ST_EXCEPTION_COINCIDENT);
-            return;
+        const int *dst = P.dst_bond + 2 * idx;
+        double fm = 0.0;
+        double3 r = make_double3(0, 0, 0);
+        if ((mask & PRISM_MD_TERM_BOND) && B.i >= 0) {
+            r = sub3(ld3d(P.x, B.j), ld3d(P.x, B.i));
+            const double len = sqrt(dot3(r, r));
+            if (len >= 1e-8) {
+                const double dr = len - (double)B.r0;
+                fm = -2.0 * (double)B.k * dr / len;
+                energy = (double)B.k * dr * dr;
+            }
         }
-        const double inv_r2 = 1.0 / r2, inv_r = sqrt(inv_r2);
-        const double s2 = (double)X.sigma * (double)X.sigma * inv_r2, s6 = s2 * s2 * s2;
-        const double qq = COULOMB * (double)X.qq;
-        const double eps = X.eps;
-        const double f = (24.0 * eps * s6 * (2.0 * s6 - 1.0) + qq * inv_r) * inv_r2;
-        add_force(P, X.i, f * d.x, f * d.y, f * d.z);
-        add_force(P, X.j, -f * d.x, -f * d.y, -f * d.z);
-        if (ENERGY) add_energy(P, X.i, 4, 4.0 * eps * (s6 * s6 - s6) + qq * inv_r);
+        if (B.i >= 0) {
+            put(P, dst[0], -fm * r.x, -fm * r.y, -fm * r.z);
+            put(P, dst[1], fm * r.x, fm * r.y, fm * r.z);
+        }
+    } else if ((idx -= P.n_bonds) < P.n_angles) {
+        const Angle A = P.angles[idx];
+        const int *dst = P.dst_angle + 3 * idx;
+        double fm = 0.0;
+        double3 di = make_double3(0, 0, 0), dk = make_double3(0, 0, 0);
+        if ((mask & PRISM_MD_TERM_ANGLE) && A.i >= 0) {
+            const double3 pj = ld3d(P.x, A.j);
+            const double3 a = sub3(ld3d(P.x, A.i), pj), b = sub3(ld3d(P.x, A.k), pj);
+            const double la = sqrt(dot3(a, a)), lb = sqrt(dot3(b, b));
+            if (la >= 1e-8 && lb >= 1e-8) {
+                double cs = dot3(a, b) / (la * lb);
+                cs = fmax(-1.0, fmin(1.0, cs));
+                const double dth = acos(cs) - (double)A.theta0;
+                double sn = sqrt(1.0 - cs * cs);
+                if (sn < 1e-8) sn = 1e-8;
+                fm = -2.0 * (double)A.kf * dth / sn;
+                const double ia = 1.0 / la, ib = 1.0 / lb;
+                di = make_double3((b.x * ib - cs * a.x * ia) * ia, (b.y * ib - cs * a.y * ia) * ia, (b.z * ib - cs * a.z * ia) * ia);
+                dk = make_double3((a.x * ia - cs * b.x * ib) * ib, (a.y * ia - cs * b.y * ib) * ib, (a.z * ia - cs * b.z * ib) * ib);
+                energy = (double)A.kf * dth * dth;
+            }
+        }
+        if (A.i >= 0) {
+            put(P, dst[0], -fm * di.x, -fm * di.y, -fm * di.z);
+            put(P, dst[1], fm * (di.x + dk.x), fm * (di.y + dk.y), fm * (di.z + dk.z));
+            put(P, dst[2], -fm * dk.x, -fm * dk.y, -fm * dk.z);
+        }
+    } else if ((idx -= P.n_angles) < P.n_dih) {
+        const Dihedral Q = P.dih[idx];
+        const int *dst = P.dst_dih + 4 * idx;
+        double3 f1 = make_double3(0, 0, 0), f4 = make_double3(0, 0, 0), sv = make_double3(0, 0, 0);
+        if ((mask & PRISM_MD_TERM_DIHEDRAL) && Q.i >= 0) {
+            const double3 pj = ld3d(P.x, Q.j), pk = ld3d(P.x, Q.k);
+            const double3 v0 = sub3(ld3d(P.x, Q.i), pj), v1 = sub3(pk, pj), v2 = sub3(pk, ld3d(P.x, Q.l));
+            const double3 c0 = cross3(v0, v1), c1 = cross3(v1, v2);
+            const double n1 = dot3(c0, c0), n2 = dot3(c1, c1), nb = dot3(v1, v1);
+            if (n1 >= 1e-16 && n2 >= 1e-16 && nb >= 1e-16) {
+                const double lb = sqrt(nb);
+                // phi from atan2 of |c0 x c1| and c0.c1: the reference's acos /
+                // asin branches (nhs_amber_fused.cu:562-580) in exact arithmetic
+                const double3 cc = cross3(c0, c1);
+                double phi = atan2(sqrt(dot3(cc, cc)), dot3(c0, c1));
+                if (dot3(v0, c1) < 0.0) phi = -phi;
+                const double nn = (double)Q.n;
+                double sa, ca;
+                sincos(nn * phi - (double)Q.phase, &sa, &ca);
+                const double dE = -(double)Q.kf * nn * sa;
+                const double ff0 = (-dE * lb) / n1, ff3 = (dE * lb) / n2;
+                const double ff1 = dot3(v0, v1) / nb, ff2 = dot3(v2, v1) / nb;
+                f1 = make_double3(ff0 * c0.x, ff0 * c0.y, ff0 * c0.z);
+                f4 = make_double3(ff3 * c1.x, ff3 * c1.y, ff3 * c1.z);
+                sv = make_double3(ff1 * f1.x - ff2 * f4.x, ff1 * f1.y - ff2 * f4.y, ff1 * f1.z - ff2 * f4.z);
+                energy = (double)Q.kf * (1.0 + ca);
+            }
+        }
+        if (Q.i >= 0) {
+            put(P, dst[0], f1.x, f1.y, f1.z);
+            put(P, dst[1], sv.x - f1.x, sv.y - f1.y, sv.z - f1.z);
+            put(P, dst[2], -sv.x - f4.x, -sv.y - f4.y, -sv.z - f4.z);
+            put(P, dst[3], f4.x, f4.y, f4.z);
+        }
+    } else if ((idx -= P.n_dih) < P.n_exc) {
+        const Exception14 X = P.exc[idx];
+        const int *dst = P.dst_exc + 2 * idx;
+        double f = 0.0;
+        double3 d = make_double3(0, 0, 0);
+        if ((mask & PRISM_MD_TERM_EXCEPTION) && X.i >= 0) {
+            d = sub3(ld3d(P.x, X.i), ld3d(P.x, X.j));
+            const double r2 = dot3(d, d);
+            if (r2 < 1e-8) {  // coincident exception atoms (nhs_amber_fused.cu:3338-3343)
+                atomicOr(&P.sst[P.atom_sys[X.i]].status, ST_EXCEPTION_COINCIDENT);
+            } else {
+                const double inv_r2 = 1.0 / r2, inv_r = sqrt(inv_r2);
+                const double s2 = (double)X.sigma * (double)X.sigma * inv_r2, s6 = s2 * s2 * s2;
+                const double qq = COULOMB * (double)X.qq;
+                const double eps = X.eps;
+                f = (24.0 * eps * s6 * (2.0 * s6 - 1.0) + qq * inv_r) * inv_r2;
+                energy = 4.0 * eps * (s6 * s6 - s6) + qq * inv_r;
+            }
+        }
+        if (X.i >= 0) {
+            put(P, dst[0], f * d.x, f * d.y, f * d.z);
+            put(P, dst[1], -f * d.x, -f * d.y, -f * d.z);
+        }
+    } else {
+        return;
+    }
+    if (ENERGY) P.term_e[tid] = energy;
+}
+
+// One thread per arena atom: the int64 fixed-point force is the sum, in unit
+// order, of the nonbonded partials of the atom's block quarter, plus the
+// atom's bonded contribution slots in slot order.
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_gather(DevPtrs P, unsigned mask) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n_atoms_arena) return;
+    const int s = P.atom_sys[a];
+    if (s < 0 || !P.sys[s].active) return;
+    long long acc[3] = {0, 0, 0};
+    double e = 0.0;
+    if (mask & PRISM_MD_TERM_NONBONDED) {
+        const int slot = P.slot_of[a];
+        const int b = slot / BLOCK, q = (slot % BLOCK) / SUB, il = slot % SUB;
+        const int per_q = P.unit_count[b] / QUARTERS;
+        const float4 *uf = P.unit_f + ((size_t)P.unit_off[b] + (size_t)q * per_q) * SUB + il;
+        for (int c = 0; c < per_q; ++c) {
+            const float4 p = uf[(size_t)c * SUB];
+            acc[0] += to_fixed(p.x, FORCE_FIXED_BITS);
+            acc[1] += to_fixed(p.y, FORCE_FIXED_BITS);
+            acc[2] += to_fixed(p.z, FORCE_FIXED_BITS);
+            if (ENERGY) e += p.w;
+        }
+    }
+    for (int k = P.inc_off[a]; k < P.inc_off[a + 1]; ++k) {
+        const float *c = P.contrib + (size_t)k * 3;
+        acc[0] += to_fixed(c[0], FORCE_FIXED_BITS);
+        acc[1] += to_fixed(c[1], FORCE_FIXED_BITS);
+        acc[2] += to_fixed(c[2], FORCE_FIXED_BITS);
+    }
+    long long *f = P.f + (size_t)a * 4;
+    f[0] = acc[0];
+    f[1] = acc[1];
+    f[2] = acc[2];
+    if (ENERGY) P.atom_e[a] = e;
+}
+
+// One workgroup per system: FP64 sums in a fixed order of the per-atom
+// nonbonded energies and the per-term energies of the last evaluation.
+__global__ __launch_bounds__(256) void k_energy_reduce(DevPtrs P) {
+    const int s = blockIdx.x;
+    if (s >= P.n_sys) return;
+    const SysDesc &D = P.sys[s];
+    __shared__ double red[256];
+    const int begins[5] = {D.bond_begin, D.angle_begin, D.dih_begin, -1, D.exc_begin};
+    const int counts[5] = {D.n_bonds, D.n_angles, D.n_dih, D.n_atoms, D.n_exc};
+    const int bases[5] = {0, P.n_bonds, P.n_bonds + P.n_angles, 0, P.n_bonds + P.n_angles + P.n_dih};
+    for (int term = 0; term < 5; ++term) {
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < counts[term]; i += blockDim.x)
+            acc += term == 3 ? P.atom_e[D.atom_begin + i] : P.term_e[bases[term] + begins[term] + i];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) P.sst[s].energy[term] = red[0];
+        __syncthreads();
     }
 }
 
@@ -377,6 +449,16 @@ void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_
     const int grid = (n + 255) / 256;
     if (energy) hipLaunchKernelGGL(k_bonded<true>, dim3(grid), dim3(256), 0, st, P, term_mask);
     else hipLaunchKernelGGL(k_bonded<false>, dim3(grid), dim3(256), 0, st, P, term_mask);
+}
+
+void launch_gather(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t st) {
+    const int grid = (P.n_atoms_arena + 255) / 256;
+    if (energy) hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(256), 0, st, P, term_mask);
+    else hipLaunchKernelGGL(k_gather<false>, dim3(grid), dim3(256), 0, st, P, term_mask);
+}
+
+void launch_energy_reduce(const DevPtrs &P, hipStream_t st) {
+    hipLaunchKernelGGL(k_energy_reduce, dim3(P.n_sys), dim3(256), 0, st, P);
 }
 
 void launch_zero_forces(const DevPtrs &P, hipStream_t st) {

@@ -20,105 +20,126 @@ namespace {
 
 constexpr int IW = 4;  // waves per workgroup
 
+// Register-resident group state.  Every loop over the group's atoms or bonds
+// below is fully unrolled with a runtime guard, so all indices are
+// compile-time constants and nothing spills to scratch memory.
 struct GroupState {
     int n;              // atoms in the group (1..4)
     int a[4];
     float m[4];
-    float3 x[4], v[4];
+    float x[4][3], v[4][3];
 };
 
-__device__ inline float3 f3(float4 p) { return make_float3(p.x, p.y, p.z); }
+#define FOR_ATOMS(k) _Pragma("unroll") for (int k = 0; k < 4; ++k) if (k < g.n)
+#define FOR_H(h) _Pragma("unroll") for (int h = 1; h < 4; ++h) if (h < g.n)
 
 // rattle_cluster_velocities (nhs_amber_fused.cu:2175-2360), FP64 iteration.
-__device__ bool rattle(GroupState &g, const GroupParam &gp, double tol, int max_iter) {
-    if (g.n < 2) return true;
-    double v[4][3], x[4][3];
-    for (int k = 0; k < g.n; ++k) {
-        x[k][0] = g.x[k].x; x[k][1] = g.x[k].y; x[k][2] = g.x[k].z;
-        v[k][0] = g.v[k].x; v[k][1] = g.v[k].y; v[k][2] = g.v[k].z;
+// Returns the sweeps used, or -1 when the iteration fails.
+__device__ int rattle(GroupState &g, const GroupParam &gp, double tol, int max_iter) {
+    if (g.n < 2) return 0;
+    double x[4][3], v[4][3];
+    FOR_ATOMS(k) {
+        for (int c = 0; c < 3; ++c) {
+            x[k][c] = g.x[k][c];
+            v[k][c] = g.v[k][c];
+        }
     }
     const double imc = gp.inv_mass_c, imh = gp.inv_mass_h;
+    int used = 0;
     bool converged = false;
-    for (int it = 0; it < max_iter && !converged; ++it) {
+    while (!converged && used < max_iter) {
+        ++used;
         double max_err = 0.0;
-        for (int h = 1; h < g.n; ++h) {
+        bool bad = false;
+        FOR_H(h) {
             const double rx = x[h][0] - x[0][0], ry = x[h][1] - x[0][1], rz = x[h][2] - x[0][2];
             const double r2 = rx * rx + ry * ry + rz * rz;
-            if (!(r2 >= 1e-12) || !isfinite(r2)) return false;
             const double ux = v[h][0] - v[0][0], uy = v[h][1] - v[0][1], uz = v[h][2] - v[0][2];
             const double res = rx * ux + ry * uy + rz * uz;
             const double rel = fabs(res) / (sqrt(r2) * fmax(sqrt(ux * ux + uy * uy + uz * uz), 1.0));
-            if (!isfinite(rel)) return false;
+            bad |= !(r2 >= 1e-12) || !isfinite(rel);
             max_err = fmax(max_err, rel);
-            if (rel <= tol) continue;
-            const double lam = res / ((imc + imh) * r2);
-            v[0][0] += imc * lam * rx; v[0][1] += imc * lam * ry; v[0][2] += imc * lam * rz;
-            v[h][0] -= imh * lam * rx; v[h][1] -= imh * lam * ry; v[h][2] -= imh * lam * rz;
+            if (rel > tol) {
+                const double lam = res / ((imc + imh) * r2);
+                v[0][0] += imc * lam * rx; v[0][1] += imc * lam * ry; v[0][2] += imc * lam * rz;
+                v[h][0] -= imh * lam * rx; v[h][1] -= imh * lam * ry; v[h][2] -= imh * lam * rz;
+            }
         }
+        if (bad) return -1;
         converged = max_err <= tol;
     }
-    for (int k = 0; k < g.n; ++k) g.v[k] = make_float3((float)v[k][0], (float)v[k][1], (float)v[k][2]);
-    return converged;
+    FOR_ATOMS(k) {
+        for (int c = 0; c < 3; ++c) g.v[k][c] = (float)v[k][c];
+    }
+    return converged ? used : -1;
 }
 
 // shake_cluster_positions (nhs_amber_fused.cu:1889-2173), FP64 iteration:
 // corrections along the step-start bond, velocity impulse correction/dt.
-__device__ bool shake(GroupState &g, const float3 xs[4], const GroupParam &gp, float dt, double tol, int max_iter) {
-    if (g.n < 2) return true;
-    double v[4][3], x[4][3];
-    for (int k = 0; k < g.n; ++k) {
-        x[k][0] = g.x[k].x; x[k][1] = g.x[k].y; x[k][2] = g.x[k].z;
-        v[k][0] = g.v[k].x; v[k][1] = g.v[k].y; v[k][2] = g.v[k].z;
+// Returns the sweeps used, or -1 when the iteration fails.
+__device__ int shake(GroupState &g, const float xs[4][3], const GroupParam &gp, float dt, double tol, int max_iter) {
+    if (g.n < 2) return 0;
+    double x[4][3], v[4][3], o[4][3];
+    FOR_ATOMS(k) {
+        for (int c = 0; c < 3; ++c) {
+            x[k][c] = g.x[k][c];
+            v[k][c] = g.v[k][c];
+        }
+    }
+    FOR_H(h) {
+        for (int c = 0; c < 3; ++c) o[h][c] = (double)xs[h][c] - (double)xs[0][c];
     }
     const double imc = gp.inv_mass_c, imh = gp.inv_mass_h, idt = 1.0 / (double)dt;
+    int used = 0;
     bool converged = false;
-    for (int it = 0; it < max_iter && !converged; ++it) {
+    while (!converged && used < max_iter) {
+        ++used;
         double max_err = 0.0;
-        for (int h = 1; h < g.n; ++h) {
+        bool bad = false;
+        FOR_H(h) {
             const double rx = x[h][0] - x[0][0], ry = x[h][1] - x[0][1], rz = x[h][2] - x[0][2];
-            const double ox = (double)xs[h].x - (double)xs[0].x, oy = (double)xs[h].y - (double)xs[0].y,
-                         oz = (double)xs[h].z - (double)xs[0].z;
             const double d = gp.d[h - 1];
             const double r2 = rx * rx + ry * ry + rz * rz, d2 = d * d, diff = r2 - d2;
-            if (!isfinite(r2) || !(d2 > 0.0)) return false;
             const double rel = fabs(diff) / d2;
+            bad |= !isfinite(r2) || !(d2 > 0.0);
             max_err = fmax(max_err, rel);
-            if (rel <= tol) continue;
-            const double rr0 = rx * ox + ry * oy + rz * oz;
-            const double scale = sqrt(r2 * (ox * ox + oy * oy + oz * oz));
-            if (!isfinite(rr0) || !(scale > 0.0) || fabs(rr0) / scale <= 1e-6) return false;
-            const double lam = diff / (2.0 * (imc + imh) * rr0);
-            x[0][0] += imc * lam * ox; x[0][1] += imc * lam * oy; x[0][2] += imc * lam * oz;
-            x[h][0] -= imh * lam * ox; x[h][1] -= imh * lam * oy; x[h][2] -= imh * lam * oz;
-            v[0][0] += imc * lam * ox * idt; v[0][1] += imc * lam * oy * idt; v[0][2] += imc * lam * oz * idt;
-            v[h][0] -= imh * lam * ox * idt; v[h][1] -= imh * lam * oy * idt; v[h][2] -= imh * lam * oz * idt;
+            if (rel > tol) {
+                const double ox = o[h][0], oy = o[h][1], oz = o[h][2];
+                const double rr0 = rx * ox + ry * oy + rz * oz;
+                const double scale = sqrt(r2 * (ox * ox + oy * oy + oz * oz));
+                bad |= !isfinite(rr0) || !(scale > 0.0) || fabs(rr0) / scale <= 1e-6;
+                const double lam = diff / (2.0 * (imc + imh) * rr0);
+                x[0][0] += imc * lam * ox; x[0][1] += imc * lam * oy; x[0][2] += imc * lam * oz;
+                x[h][0] -= imh * lam * ox; x[h][1] -= imh * lam * oy; x[h][2] -= imh * lam * oz;
+                v[0][0] += imc * lam * ox * idt; v[0][1] += imc * lam * oy * idt; v[0][2] += imc * lam * oz * idt;
+                v[h][0] -= imh * lam * ox * idt; v[h][1] -= imh * lam * oy * idt; v[h][2] -= imh * lam * oz * idt;
+            }
         }
+        if (bad) return -1;
         converged = max_err <= tol;
     }
-    for (int k = 0; k < g.n; ++k) {
-        g.x[k] = make_float3((float)x[k][0], (float)x[k][1], (float)x[k][2]);
-        g.v[k] = make_float3((float)v[k][0], (float)v[k][1], (float)v[k][2]);
+    FOR_ATOMS(k) {
+        for (int c = 0; c < 3; ++c) {
+            g.x[k][c] = (float)x[k][c];
+            g.v[k][c] = (float)v[k][c];
+        }
     }
-    return converged;
+    return converged ? used : -1;
 }
 
 // langevin_thermostat with dt/2 (nhs_amber_fused.cu:1713-1744, :3656-3666).
 __device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float gamma, unsigned long long step,
                                int purpose, const float *noise) {
     const float c1 = expf(-gamma * (0.5f * D.dt));
-    for (int k = 0; k < g.n; ++k) {
+    FOR_ATOMS(k) {
         float xi[3];
         if (noise) {
-            xi[0] = noise[(size_t)g.a[k] * 3 + 0];
-            xi[1] = noise[(size_t)g.a[k] * 3 + 1];
-            xi[2] = noise[(size_t)g.a[k] * 3 + 2];
+            for (int c = 0; c < 3; ++c) xi[c] = noise[(size_t)g.a[k] * 3 + c];
         } else {
             langevin_normals(D.seed_lo, D.stream, (uint32_t)(g.a[k] - D.atom_begin), step, purpose, xi);
         }
         const float c2 = sqrtf((1.0f - c1 * c1) * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T / g.m[k]);
-        g.v[k].x = c1 * g.v[k].x + c2 * xi[0];
-        g.v[k].y = c1 * g.v[k].y + c2 * xi[1];
-        g.v[k].z = c1 * g.v[k].z + c2 * xi[2];
+        for (int c = 0; c < 3; ++c) g.v[k][c] = c1 * g.v[k][c] + c2 * xi[c];
     }
 }
 
@@ -126,19 +147,17 @@ __device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float g
 // (prism_asc_decision.cuh:100-104) when gamma != 0, exact when gamma == 0
 // (nhs_amber_fused.cu:1750-1805, :3667-3703).
 __device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, float gamma) {
-    for (int k = 0; k < g.n; ++k) {
+    FOR_ATOMS(k) {
         const long long *fa = P.f + (size_t)g.a[k] * 4;
-        float fx = from_fixed(fa[0], FORCE_FIXED_BITS), fy = from_fixed(fa[1], FORCE_FIXED_BITS),
-              fz = from_fixed(fa[2], FORCE_FIXED_BITS);
+        float f[3];
+        for (int c = 0; c < 3; ++c) f[c] = from_fixed(fa[c], FORCE_FIXED_BITS);
         if (gamma != 0.0f) {
-            const float mag = sqrtf(fx * fx + fy * fy + fz * fz);
+            const float mag = sqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
             const float q = mag > MAX_FORCE ? MAX_FORCE / mag : 1.0f;
-            fx *= q; fy *= q; fz *= q;
+            for (int c = 0; c < 3; ++c) f[c] *= q;
         }
         const float kick = 0.5f * dt * (float)KCAL_TO_AMU_A2_PS2 / g.m[k];
-        g.v[k].x += kick * fx;
-        g.v[k].y += kick * fy;
-        g.v[k].z += kick * fz;
+        for (int c = 0; c < 3; ++c) g.v[k][c] += kick * f[c];
     }
 }
 
@@ -147,18 +166,39 @@ __device__ inline bool load_group(const DevPtrs &P, int gidx, GroupState &g, Gro
     if (G.atom[0] < 0) return false;
     gp = P.gparams[gidx];
     g.n = 1 + gp.n_h;
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
         g.a[k] = G.atom[k];
         if (k < g.n) {
-            g.m[k] = P.mass[G.atom[k]];
-            g.x[k] = f3(P.x[G.atom[k]]);
-            g.v[k] = f3(P.v[G.atom[k]]);
+            const int a = G.atom[k];
+            g.m[k] = P.mass[a];
+            const float4 x = P.x[a], v = P.v[a];
+            g.x[k][0] = x.x; g.x[k][1] = x.y; g.x[k][2] = x.z;
+            g.v[k][0] = v.x; g.v[k][1] = v.y; g.v[k][2] = v.z;
         }
     }
     return true;
 }
 
-__device__ inline bool finite3(float3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+__device__ inline bool finite3(const float a[3]) { return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]); }
+
+// Latch the first failure of a system: status bit plus kind, atom and step.
+__device__ inline void record_fault(SysState &S, int kind, int atom, unsigned long long step) {
+    atomicOr(&S.status, kind == FAULT_NONFINITE ? ST_NONFINITE : ST_CONSTRAINT);
+    if (atomicCAS(&S.fault_atom, -1, atom) == -1) {
+        S.fault_kind = kind;
+        S.fault_step = step;
+    }
+}
+__device__ inline void note_sweeps(SysState &S, int sweeps, int kind, int atom, unsigned long long step, int &most) {
+    if (sweeps < 0) record_fault(S, kind, atom, step);
+    else most = max(most, sweeps);
+}
+// One atomicMax per wave for the sweep statistics (called in converged control flow).
+__device__ inline void publish_sweeps(int *dst, int most) {
+    for (int o = 32; o > 0; o >>= 1) most = max(most, __shfl_xor(most, o));
+    if ((threadIdx.x & 63) == 0 && most > 0) atomicMax(dst, most);
+}
 
 __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *noise) {
     const int lane = threadIdx.x & 63;
@@ -175,46 +215,44 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
     const bool com = D.com_every > 0 && step > 0 && step % (unsigned long long)D.com_every == 0;
     GroupState g;
     GroupParam gp;
-    bool ok = true, moved = false;
+    bool moved = false;
+    int most_sh = 0, most_ra = 0;
     if (load_group(P, gw * 64 + lane, g, gp)) {
-        if (com)
-            for (int k = 0; k < g.n; ++k) {
-                g.v[k].x -= (float)S.com_v[0];
-                g.v[k].y -= (float)S.com_v[1];
-                g.v[k].z -= (float)S.com_v[2];
+        if (com) {
+            FOR_ATOMS(k) {
+                for (int c = 0; c < 3; ++c) g.v[k][c] -= (float)S.com_v[c];
             }
-        if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 1, noise);
-        ok &= rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
-        half_kick(g, P, D.dt, gamma);
-        ok &= rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
-        float3 xs[4];
-        for (int k = 0; k < g.n; ++k) {
-            xs[k] = g.x[k];
-            g.x[k].x += D.dt * g.v[k].x;
-            g.x[k].y += D.dt * g.v[k].y;
-            g.x[k].z += D.dt * g.v[k].z;
         }
-        ok &= shake(g, xs, gp, D.dt, D.shake_tol, D.constraint_max_iter);
-        for (int k = 0; k < g.n; ++k) {
+        if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 1, noise);
+        note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_1, g.a[0], step, most_ra);
+        half_kick(g, P, D.dt, gamma);
+        note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_2, g.a[0], step, most_ra);
+        float xs[4][3];
+        FOR_ATOMS(k) {
+            for (int c = 0; c < 3; ++c) {
+                xs[k][c] = g.x[k][c];
+                g.x[k][c] += D.dt * g.v[k][c];
+            }
+        }
+        note_sweeps(S, shake(g, xs, gp, D.dt, D.shake_tol, D.constraint_max_iter), FAULT_SHAKE, g.a[0], step, most_sh);
+        FOR_ATOMS(k) {
             const int a = g.a[k];
-            ok &= finite3(g.x[k]) && finite3(g.v[k]);
-            P.x[a] = make_float4(g.x[k].x, g.x[k].y, g.x[k].z, 0.0f);
-            P.v[a] = make_float4(g.v[k].x, g.v[k].y, g.v[k].z, 0.0f);
-            P.xs[a] = make_float4(xs[k].x, xs[k].y, xs[k].z, 0.0f);
-            long long *fa = P.f + (size_t)a * 4;
-            fa[0] = 0; fa[1] = 0; fa[2] = 0;
-            *(float3 *)&P.xq[P.slot_of[a]] = g.x[k];
+            if (!(finite3(g.x[k]) && finite3(g.v[k]))) record_fault(S, FAULT_NONFINITE, a, step);
+            P.x[a] = make_float4(g.x[k][0], g.x[k][1], g.x[k][2], 0.0f);
+            P.v[a] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
+            P.xs[a] = make_float4(xs[k][0], xs[k][1], xs[k][2], 0.0f);
+            float *q = (float *)&P.xq[P.slot_of[a]];
+            q[0] = g.x[k][0]; q[1] = g.x[k][1]; q[2] = g.x[k][2];
             const float4 b = P.x_build[a];
-            const float dx = g.x[k].x - b.x, dy = g.x[k].y - b.y, dz = g.x[k].z - b.z;
+            const float dx = g.x[k][0] - b.x, dy = g.x[k][1] - b.y, dz = g.x[k][2] - b.z;
             moved |= dx * dx + dy * dy + dz * dz > D.half_skin2;
         }
     }
+    publish_sweeps(&S.max_shake_iter, most_sh);
+    publish_sweeps(&S.max_rattle_iter, most_ra);
     const bool due = (long long)step - S.rebuild_step >= (long long)D.rebuild_every;
     if (__ballot(moved) != 0ull || due) {
         if (lane == 0) S.rebuild_flag = 1;
-    }
-    if (__ballot(!ok) != 0ull && lane == 0) {
-        atomicOr(&S.status, ST_CONSTRAINT);
     }
 }
 
@@ -232,36 +270,34 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         const bool next_com = D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0;
         GroupState g;
         GroupParam gp;
-        bool ok = true;
         double mv[3] = {0.0, 0.0, 0.0};
+        int most_ra = 0;
         if (load_group(P, gw * 64 + lane, g, gp)) {
             half_kick(g, P, D.dt, gamma);
-            ok &= rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_3, g.a[0], step, most_ra);
             if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 2, noise);
-            for (int k = 0; k < g.n; ++k) {  // nhs_velocity_safety_clamp (nhs_amber_fused.cu:11148-11186)
+            FOR_ATOMS(k) {  // nhs_velocity_safety_clamp (nhs_amber_fused.cu:11148-11186)
                 const float thermal = sqrtf((float)KCAL_TO_AMU_A2_PS2 * (float)KB * T / fmaxf(g.m[k], 1.0f));
                 const float vmax = fmaxf(20.0f * thermal, 2.0f);
-                const float sp = sqrtf(g.v[k].x * g.v[k].x + g.v[k].y * g.v[k].y + g.v[k].z * g.v[k].z);
+                const float sp = sqrtf(g.v[k][0] * g.v[k][0] + g.v[k][1] * g.v[k][1] + g.v[k][2] * g.v[k][2]);
                 if (sp > vmax) {
                     const float sc = vmax / sp;
-                    g.v[k].x *= sc; g.v[k].y *= sc; g.v[k].z *= sc;
+                    for (int c = 0; c < 3; ++c) g.v[k][c] *= sc;
                 }
             }
-            ok &= rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
-            for (int k = 0; k < g.n; ++k) {
-                ok &= finite3(g.v[k]);
-                P.v[g.a[k]] = make_float4(g.v[k].x, g.v[k].y, g.v[k].z, 0.0f);
-                mv[0] += (double)g.m[k] * g.v[k].x;
-                mv[1] += (double)g.m[k] * g.v[k].y;
-                mv[2] += (double)g.m[k] * g.v[k].z;
+            note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_4, g.a[0], step, most_ra);
+            FOR_ATOMS(k) {
+                if (!finite3(g.v[k])) record_fault(S, FAULT_NONFINITE, g.a[k], step);
+                P.v[g.a[k]] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
+                for (int c = 0; c < 3; ++c) mv[c] += (double)g.m[k] * g.v[k][c];
             }
         }
+        publish_sweeps(&S.max_rattle_iter, most_ra);
         if (next_com) {
             for (int c = 0; c < 3; ++c) mv[c] = wave_sum(mv[c]);
             if (lane == 0)
                 for (int c = 0; c < 3; ++c) P.com_partial[(size_t)gw * 3 + c] = mv[c];
         }
-        if (__ballot(!ok) != 0ull && lane == 0) atomicOr(&S.status, ST_CONSTRAINT);
     }
     // Last-block handshake (no waiting): the workgroup that retires last sums
     // the COM partials in wave order and advances every system's step counter.

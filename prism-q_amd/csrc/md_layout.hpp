@@ -19,7 +19,7 @@ namespace prism {
 
 constexpr int BLOCK = 64;            // atoms per spatial block (= wavefront)
 constexpr int MAX_ROW = 512;         // j-blocks per i-block row (full list)
-constexpr int MAX_MASKS = 32;        // exclusion-mask tiles per row
+constexpr int MAX_MASKS = 64;        // exclusion-mask tiles per row
 constexpr int SUB = 16;              // i-atoms per nonbonded wave (block quarter)
 constexpr int QUARTERS = BLOCK / SUB;
 constexpr int CHUNK = 8;             // tiles per nonbonded work unit
@@ -67,6 +67,9 @@ struct SysDesc {
     uint32_t seed_lo, stream;
     int noise_explicit;             // 1: read normals from the explicit array
     int pad1;
+    // term ranges of this system in the global term arrays (energy sums)
+    int bond_begin, n_bonds, angle_begin, n_angles;
+    int dih_begin, n_dih, exc_begin, n_exc;
 };
 
 // Mutable per-system state (device-written).
@@ -78,8 +81,23 @@ struct SysState {
     int status;                     // ST_* bits
     int n_rebuilds;
     double com_v[3];                // centre-of-mass velocity for the next COM step
-    long long energy[8];            // fixed-point term energies (5 used)
+    double energy[8];               // FP64 term energies of the last evaluation (5 used)
     unsigned long long force_evals;
+    int fault_atom;                 // first atom of the first failing group (-1: none)
+    int fault_kind;                 // FAULT_* of that failure
+    unsigned long long fault_step;
+    int max_shake_iter;             // most SHAKE sweeps any group needed
+    int max_rattle_iter;            // most RATTLE sweeps any group needed
+};
+
+// failure kinds recorded with the first fault
+enum : int {
+    FAULT_RATTLE_1 = 1,     // RATTLE after the first OU half-step
+    FAULT_RATTLE_2 = 2,     // RATTLE before the drift
+    FAULT_SHAKE = 3,        // SHAKE of the drifted positions
+    FAULT_RATTLE_3 = 4,     // RATTLE before the second OU half-step
+    FAULT_RATTLE_4 = 5,     // final RATTLE
+    FAULT_NONFINITE = 6,    // non-finite position or velocity
 };
 
 // Global engine counters (device-written).

@@ -73,7 +73,8 @@ class ParamsV1(C.Structure):
 
 class StatsV1(C.Structure):
     _fields_ = [("steps_done", C.c_uint64), ("pair_rebuilds", C.c_uint64), ("force_evaluations", C.c_uint64),
-                ("n_tiles", C.c_int32), ("n_blocks", C.c_int32), ("status", C.c_int32), ("reserved", C.c_int32)]
+                ("n_tiles", C.c_int32), ("n_blocks", C.c_int32), ("status", C.c_int32),
+                ("max_shake_sweeps", C.c_int32), ("max_rattle_sweeps", C.c_int32), ("reserved", C.c_int32)]
 
 
 class GenParamsV1(C.Structure):
