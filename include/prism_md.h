@@ -156,7 +156,9 @@ typedef struct PrismMdParamsV1 {
     int32_t  warm_tail;           /* 1: hold end_temperature after ramp_down_end      */
     int32_t  equilibration_steps; /* gamma ramp length                                */
     double   equilibration_gamma_ps; /* gamma at step 0 of the ramp                   */
-    int32_t  reserved[8];
+    int32_t  resort_every;        /* re-sort atoms into blocks every Nth rebuild (8);
+                                   * other rebuilds refresh block boxes and rows only */
+    int32_t  reserved[7];
 } PrismMdParamsV1;
 
 typedef struct PrismMdStatsV1 {

@@ -6,6 +6,10 @@
 
 namespace prism {
 
+__device__ inline int wave_min_int(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
 __device__ inline float wave_min(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
     return v;

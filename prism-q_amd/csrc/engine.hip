@@ -318,6 +318,7 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
         D.com_every = p.com_remove_every;
         D.constraint_max_iter = p.constraint_max_iter;
         D.sort_bits = 10;
+        D.resort_every = std::max(1, p.resort_every);
         D.dt = (float)p.dt_ps;
         D.cutoff = (float)p.cutoff_A;
         D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);
@@ -399,6 +400,8 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
     P_.row_j = dalloc<int>((size_t)B * MAX_ROW);
     P_.row_m = dalloc<int>((size_t)B * MAX_ROW);
     P_.masks = dalloc<unsigned long long>((size_t)B * MAX_MASKS * 64);
+    P_.pblk = dalloc<int>((size_t)B * MAX_MASKS);
+    P_.n_pblk = dalloc<int>(B);
     P_.unit_count = dalloc<int>(B);
     P_.units = dalloc<int>((size_t)B * UNITS_PER_BLOCK);
     P_.unit_off = dalloc<int>(B);
@@ -465,15 +468,16 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
 
 // Rebuild the lists at the current positions and recompute the carried force.
 int Engine::refresh_all() {
+    static const int ones[2] = {1, 1};  // rebuild_flag, resort_flag
+    static_assert(offsetof(SysState, resort_flag) == offsetof(SysState, rebuild_flag) + sizeof(int), "flag layout");
     for (int s = 0; s < (int)sys_.size(); ++s) {
-        const int one = 1;
-        if (int rc = hipcheck(hipMemcpyAsync((char *)(P_.sst + s) + offsetof(SysState, rebuild_flag), &one, sizeof(int),
+        if (int rc = hipcheck(hipMemcpyAsync((char *)(P_.sst + s) + offsetof(SysState, rebuild_flag), ones, sizeof(ones),
                                              hipMemcpyHostToDevice, stream_),
                               "flag"))
             return rc;
     }
-    launch_sort(P_, -1, stream_);
-    launch_rows(P_, stream_);
+    launch_sort(P_, stream_);
+    launch_rows(P_, -1, stream_);
     full_force();
     return hipcheck(hipGetLastError(), "refresh");
 }
@@ -487,8 +491,8 @@ void Engine::full_force() {
 
 void Engine::launch_step(const float *noise_n, const float *noise_np1) {
     launch_first_half(P_, noise_n, stream_);
-    launch_sort(P_, 0, stream_);
-    launch_rows(P_, stream_);
+    launch_sort(P_, stream_);
+    launch_rows(P_, 0, stream_);
     launch_nonbonded(P_, false, stream_);
     launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
     launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
@@ -796,8 +800,8 @@ int Engine::minimize(int max_iter, double f_tol, double *e_final, double *f_max)
             alpha = 0.1;
         }
         launch_fire_update(P_, d_vmin_, cv, cf, (float)dt, 0.1f, stream_);
-        launch_sort(P_, -1, stream_);
-        launch_rows(P_, stream_);
+        launch_sort(P_, stream_);
+        launch_rows(P_, -1, stream_);
     }
     launch_snap_hydrogens(P_, stream_);
     if (int rc = refresh_all()) return rc;
@@ -867,7 +871,7 @@ int prism_md_default_params(PrismMdParamsV1 *p) {
     p->dt_ps = 0.004;
     p->temperature_K = 300.0;
     p->end_temperature_K = 300.0;
-    p->gamma_ps = 1.0;
+    p->gamma_ps = 10.0;  // gamma_base, fused_engine.rs:13314
     p->cutoff_A = 10.0;
     p->skin_A = 2.0;
     p->rebuild_every = 20;
@@ -877,7 +881,9 @@ int prism_md_default_params(PrismMdParamsV1 *p) {
     p->constraint_max_iter = 100;
     p->shake_tol = 1e-10;
     p->rattle_tol = 1e-12;
+    p->equilibration_steps = 10000;  // fused_engine.rs:15370-15378
     p->equilibration_gamma_ps = 1000.0;
+    p->resort_every = 8;
     return PRISM_MD_OK;
 }
 

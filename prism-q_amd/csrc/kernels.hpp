@@ -33,7 +33,9 @@ struct DevPtrs {
     const int *dst_bond, *dst_angle, *dst_dih, *dst_exc;  // first slot per term position: [n][arity]
     double *term_e;                 // per-term energy (bonds, angles, dihedrals, exceptions)
     double *atom_e;                 // per-atom nonbonded energy of the last gather
-    unsigned long long *masks;      // [B][MAX_MASKS][64]
+    unsigned long long *masks;      // [B][MAX_MASKS][64] per partner block: skip bits of each i-lane
+    int *pblk;                      // [B][MAX_MASKS] partner blocks of each block, ascending
+    int *n_pblk;                    // [B]
     // partner lists over arena atoms
     const int *skip_off, *skip_idx; // exclusions U exception partners (nonbonded skip)
     const int *ex_off, *ex_idx;     // exclusions minus exception partners (pair list)
@@ -64,10 +66,11 @@ struct RefGrid {                    // reference cell grid of the pair-list pari
     int nx, ny, nz;
 };
 
-// step_offset: 0 inside a step (lists built at x_(s+1) after SHAKE of step s),
-// -1 for a rebuild between steps (lists built at x_s).
-void launch_sort(const DevPtrs &P, int step_offset, hipStream_t s);
-void launch_rows(const DevPtrs &P, hipStream_t s);
+// Rebuild of the systems whose rebuild flag is set: re-sort or box refresh,
+// then rows and the unit list.  step_offset: 0 inside a step (lists built at
+// x_(s+1) after SHAKE of step s), -1 for a rebuild between steps (at x_s).
+void launch_sort(const DevPtrs &P, hipStream_t s);
+void launch_rows(const DevPtrs &P, int step_offset, hipStream_t s);
 void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s);
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);

@@ -49,24 +49,55 @@ __device__ inline unsigned int hilbert_key(unsigned int x, unsigned int y, unsig
     return key;
 }
 
-// One workgroup per system.  Runs only for systems whose rebuild flag is set.
-__global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int step_offset) {
-    const int s = blockIdx.x;
-    if (s >= P.n_sys) return;
-    const SysDesc &D = P.sys[s];
-    SysState &S = P.sst[s];
-    __shared__ int go;
+// Box of one block from the current positions of its atoms; the build-time
+// positions are recorded for the displacement trigger.
+__device__ inline void refresh_block(const DevPtrs &P, int b, int lane) {
+    const int a = P.perm[b * BLOCK + lane];
+    float l[3], h[3];
+    if (a >= 0) {
+        const float4 p = P.x[a];
+        P.x_build[a] = p;
+        l[0] = h[0] = p.x; l[1] = h[1] = p.y; l[2] = h[2] = p.z;
+    } else {
+        l[0] = l[1] = l[2] = 3.0e38f;
+        h[0] = h[1] = h[2] = -3.0e38f;
+    }
+    for (int c = 0; c < 3; ++c) {
+        l[c] = wave_min(l[c]);
+        h[c] = wave_max(h[c]);
+    }
+    if (lane == 0) {
+        P.bb_c[b] = make_float4(0.5f * (l[0] + h[0]), 0.5f * (l[1] + h[1]), 0.5f * (l[2] + h[2]), 0.0f);
+        P.bb_h[b] = make_float4(0.5f * (h[0] - l[0]), 0.5f * (h[1] - l[1]), 0.5f * (h[2] - l[2]), 0.0f);
+    }
+}
+
+// Workgroups [0, n_sys): one per system, a full re-sort when the system's
+// rebuild is due and resort_due() holds.  Workgroups from n_sys on: one wave
+// per block, a box refresh of blocks whose system rebuilds without re-sorting.
+// Nothing here writes the flags it reads; the row builder consumes them.
+__global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
     __shared__ float red[6][SORT_WAVES];
     __shared__ unsigned int hist[256];
     __shared__ unsigned int running[256];
     __shared__ unsigned int wave_cnt[SORT_WAVES][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) {
-        go = D.active && S.rebuild_flag;
-        S.rows_dirty = go;
+    if ((int)blockIdx.x >= P.n_sys) {
+        const int b = ((int)blockIdx.x - P.n_sys) * SORT_WAVES + wave;
+        if (b >= P.n_blocks_arena) return;
+        const int s = P.block_sys[b];
+        if (s < 0) return;
+        const SysDesc &D = P.sys[s];
+        const SysState &S = P.sst[s];
+        if (!D.active || !S.rebuild_flag || resort_due(D, S)) return;
+        refresh_block(P, b, lane);
+        if (lane == 0 && b == D.block_begin) atomicExch(&P.counters->any_dirty, 1);
+        return;
     }
-    __syncthreads();
-    if (!go) return;
+    const int s = blockIdx.x;
+    const SysDesc &D = P.sys[s];
+    const SysState &S = P.sst[s];
+    if (!D.active || !S.rebuild_flag || !resort_due(D, S)) return;
     const int n = D.n_atoms, a0 = D.atom_begin;
 
     // ---- bounding box of the current positions
@@ -181,7 +212,6 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int step_offse
             P.slot_of[a] = slot;
             P.xq[slot] = make_float4(p.x, p.y, p.z, P.charge[a] * qscale);
             P.lj[slot] = make_float2(0.5f * P.sigma[a], sqrtf(24.0f * P.eps[a]));
-            P.x_build[a] = p;
         } else {
             P.perm[slot] = -1;
             P.xq[slot] = make_float4(DUMMY_FAR + 64.0f * (float)i, DUMMY_FAR, DUMMY_FAR, 0.0f);
@@ -190,38 +220,17 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int step_offse
     }
     __syncthreads();
 
-    // ---- block bounding boxes over real atoms
-    for (int b = wave; b < D.n_blocks; b += SORT_WAVES) {
-        const int i = b * BLOCK + lane;
-        float l[3], h[3];
-        if (i < n) {
-            const float4 p = P.xq[slot0 + i];
-            l[0] = h[0] = p.x; l[1] = h[1] = p.y; l[2] = h[2] = p.z;
-        } else {
-            l[0] = l[1] = l[2] = 3.0e38f;
-            h[0] = h[1] = h[2] = -3.0e38f;
-        }
-        for (int c = 0; c < 3; ++c) {
-            l[c] = wave_min(l[c]);
-            h[c] = wave_max(h[c]);
-        }
-        if (lane == 0) {
-            P.bb_c[D.block_begin + b] = make_float4(0.5f * (l[0] + h[0]), 0.5f * (l[1] + h[1]), 0.5f * (l[2] + h[2]), 0.0f);
-            P.bb_h[D.block_begin + b] = make_float4(0.5f * (h[0] - l[0]), 0.5f * (h[1] - l[1]), 0.5f * (h[2] - l[2]), 0.0f);
-        }
-    }
-    if (tid == 0) {
-        S.rebuild_flag = 0;
-        S.rebuild_step = (long long)S.step + step_offset;
-        S.n_rebuilds += 1;
-        atomicExch(&P.counters->any_dirty, 1);
-    }
+    // ---- block boxes over real atoms
+    for (int b = wave; b < D.n_blocks; b += SORT_WAVES) refresh_block(P, D.block_begin + b, lane);
+    if (tid == 0) atomicExch(&P.counters->any_dirty, 1);
 }
 
 // One wave per i-block: the row of j-blocks whose build-time boxes lie within
-// the list cutoff (full list, both directions, ascending j), then the
-// exclusion masks of the row's tiles that hold a skipped pair.
-__device__ void build_row(const DevPtrs &P, int b, int wave, int lane, int (*row_lds)[MAX_ROW], int (*flag_lds)[MAX_ROW]) {
+// the list cutoff (full list, both directions, ascending j) and the mask slot
+// of each tile whose j-block holds a skipped pair (or is the block itself).
+// The masks themselves are rebuilt only when the atoms were re-sorted.
+__device__ void build_row(const DevPtrs &P, int b, bool resort, int wave, int lane, int (*row_lds)[MAX_ROW],
+                          int (*flag_lds)[MAX_ROW], unsigned long long (*mask_lds)[MAX_MASKS][64]) {
     const int s = P.block_sys[b];
     const SysDesc &D = P.sys[s];
     SysState &S = P.sst[s];
@@ -257,74 +266,91 @@ __device__ void build_row(const DevPtrs &P, int b, int wave, int lane, int (*row
         P.row_n[b] = count;
         P.unit_count[b] = QUARTERS * ((count + CHUNK - 1) / CHUNK);
     }
-    for (int t = lane; t < MAX_ROW; t += 64) flag_lds[wave][t] = 0;
+    int *pb = flag_lds[wave];  // the block's partner blocks, ascending
+    int n_pb = 0;
+    if (resort) {
+        // Partner blocks: the block itself (self pairs) and every block holding
+        // a skip partner of one of its atoms, found by repeated wave minimum;
+        // then one 64-bit mask word per (partner block, i-lane).  These change
+        // only when the atoms are re-sorted.
+        const int a = P.perm[b * BLOCK + lane];
+        const int e0 = a >= 0 ? P.skip_off[a] : 0, e1 = a >= 0 ? P.skip_off[a + 1] : 0;
+        int cur = -1;
+        for (;;) {
+            int cand = cur < b ? b : 0x7fffffff;
+            for (int e = e0; e < e1; ++e) {
+                const int q = P.slot_of[P.skip_idx[e]] / BLOCK;
+                if (q > cur && q < cand) cand = q;
+            }
+            cand = wave_min_int(cand);
+            if (cand == 0x7fffffff) break;
+            if (n_pb < MAX_MASKS && lane == 0) pb[n_pb] = cand;
+            ++n_pb;
+            cur = cand;
+        }
+        if (n_pb > MAX_MASKS) {
+            if (lane == 0) atomicOr(&S.status, ST_MASK_OVERFLOW);
+            n_pb = MAX_MASKS;
+        }
+        wave_lds_fence();
+        auto slot_in = [&](int q) -> int {  // index of block q in pb, -1 if absent
+            int lo = 0, hi2 = n_pb - 1;
+            while (lo <= hi2) {
+                const int mid = (lo + hi2) >> 1;
+                const int v = pb[mid];
+                if (v == q) return mid;
+                if (v < q) lo = mid + 1; else hi2 = mid - 1;
+            }
+            return -1;
+        };
+        // each lane owns column `lane` of the mask words: no atomics
+        for (int m = 0; m < n_pb; ++m) mask_lds[wave][m][lane] = 0ull;
+        const int mb = slot_in(b);
+        if (mb >= 0) mask_lds[wave][mb][lane] = 1ull << lane;  // self pair
+        for (int e = e0; e < e1; ++e) {
+            const int ps = P.slot_of[P.skip_idx[e]];
+            const int m = slot_in(ps / BLOCK);
+            if (m >= 0) mask_lds[wave][m][lane] |= 1ull << (ps % BLOCK);
+        }
+        unsigned long long *mrow = P.masks + (size_t)b * MAX_MASKS * 64;
+        for (int m = 0; m < n_pb; ++m) mrow[m * 64 + lane] = mask_lds[wave][m][lane];
+        if (lane < n_pb) P.pblk[(size_t)b * MAX_MASKS + lane] = pb[lane];
+        if (lane == 0) P.n_pblk[b] = n_pb;
+    } else {
+        n_pb = P.n_pblk[b];
+        if (lane < n_pb) pb[lane] = P.pblk[(size_t)b * MAX_MASKS + lane];
+    }
     wave_lds_fence();
-
-    // tiles holding skipped pairs (and the diagonal tile, for the self pair)
-    const int slot = b * BLOCK + lane;
-    const int a = P.perm[slot];
-    auto find = [&](int jb) -> int {
-        int lo = 0, hi2 = count - 1;
+    // mask slot of every tile of the row: its j-block's place among the
+    // partner blocks (both lists ascending)
+    for (int t = lane; t < count; t += 64) {
+        const int jb = row_lds[wave][t];
+        int lo = 0, hi2 = n_pb - 1, m = -1;
         while (lo <= hi2) {
             const int mid = (lo + hi2) >> 1;
-            const int v = row_lds[wave][mid];
-            if (v == jb) return mid;
+            const int v = pb[mid];
+            if (v == jb) { m = mid; break; }
             if (v < jb) lo = mid + 1; else hi2 = mid - 1;
         }
-        return -1;
-    };
-    const int tdiag = find(b);
-    if (lane == 0 && tdiag >= 0) flag_lds[wave][tdiag] = 1;
-    if (a >= 0)
-        for (int e = P.skip_off[a]; e < P.skip_off[a + 1]; ++e) {
-            const int t = find(P.slot_of[P.skip_idx[e]] / BLOCK);
-            if (t >= 0) atomicOr(&flag_lds[wave][t], 1);
-        }
-    wave_lds_fence();
-    int n_masks = 0;
-    for (int t0 = 0; t0 < count; t0 += 64) {
-        const int t = t0 + lane;
-        const bool f = t < count && flag_lds[wave][t];
-        const unsigned long long m = __ballot(f);
-        const int ms = n_masks + __popcll(m & lt);
-        if (t < count) {
-            const int idx = (f && ms < MAX_MASKS) ? ms : -1;
-            P.row_m[(size_t)b * MAX_ROW + t] = idx;
-            flag_lds[wave][t] = idx;  // tile -> mask slot
-        }
-        n_masks += __popcll(m);
+        P.row_m[(size_t)b * MAX_ROW + t] = m;
     }
-    if (n_masks > MAX_MASKS && lane == 0) atomicOr(&S.status, ST_MASK_OVERFLOW);
-    wave_lds_fence();
-    unsigned long long *mrow = P.masks + (size_t)b * MAX_MASKS * 64;
-    for (int m = 0; m < min(n_masks, MAX_MASKS); ++m) mrow[m * 64 + lane] = 0ull;
-    if (tdiag >= 0) {
-        const int ms = flag_lds[wave][tdiag];
-        if (ms >= 0) mrow[ms * 64 + lane] = 1ull << lane;  // self pair
-    }
-    if (a >= 0)
-        for (int e = P.skip_off[a]; e < P.skip_off[a + 1]; ++e) {
-            const int ps = P.slot_of[P.skip_idx[e]];
-            const int t = find(ps / BLOCK);
-            if (t < 0) continue;
-            const int ms = flag_lds[wave][t];
-            if (ms >= 0) mrow[ms * 64 + lane] |= 1ull << (ps % BLOCK);
-        }
 }
 
-constexpr int ROW_WAVES = 4;
-__global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
+constexpr int ROW_WAVES = 2;
+__global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P, int step_offset) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = blockIdx.x * ROW_WAVES + wave;
     __shared__ int row_lds[ROW_WAVES][MAX_ROW];
     __shared__ int flag_lds[ROW_WAVES][MAX_ROW];
+    __shared__ unsigned long long mask_lds[ROW_WAVES][MAX_MASKS][64];
     __shared__ int last;
     // the sort of this step raised any_dirty when it re-sorted some system;
     // without it no row changes and the unit list stays valid
     if (P.counters->any_dirty == 0) return;
     if (b < P.n_blocks_arena) {
         const int s = P.block_sys[b];
-        if (s >= 0 && P.sys[s].active && P.sst[s].rows_dirty) build_row(P, b, wave, lane, row_lds, flag_lds);
+        if (s >= 0 && P.sys[s].active && P.sst[s].rebuild_flag)
+            build_row(P, b, resort_due(P.sys[s], P.sst[s]), wave, lane, row_lds, flag_lds, mask_lds);
     }
     // Last-block handshake (no waiting): when any row changed, the workgroup
     // that retires last rebuilds the compact list of nonbonded work units,
@@ -372,18 +398,28 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
             P.counters->any_dirty = 0;
         }
     }
+    // every other workgroup has finished reading the flags: consume them
+    for (int s = threadIdx.x; s < P.n_sys; s += blockDim.x) {
+        SysState &S = P.sst[s];
+        if (!P.sys[s].active || !S.rebuild_flag) continue;
+        S.rebuild_flag = 0;
+        S.resort_flag = 0;
+        S.rebuild_step = (long long)S.step + step_offset;
+        S.n_rebuilds += 1;
+    }
     if (threadIdx.x == 0) P.counters->rows_done = 0;
 }
 
 }  // namespace
 
-void launch_sort(const DevPtrs &P, int step_offset, hipStream_t st) {
-    hipLaunchKernelGGL(k_sort, dim3(P.n_sys), dim3(SORT_THREADS), 0, st, P, step_offset);
+void launch_sort(const DevPtrs &P, hipStream_t st) {
+    const int refresh = (P.n_blocks_arena + SORT_WAVES - 1) / SORT_WAVES;
+    hipLaunchKernelGGL(k_sort, dim3(P.n_sys + refresh), dim3(SORT_THREADS), 0, st, P);
 }
 
-void launch_rows(const DevPtrs &P, hipStream_t st) {
+void launch_rows(const DevPtrs &P, int step_offset, hipStream_t st) {
     const int grid = (P.n_blocks_arena + ROW_WAVES - 1) / ROW_WAVES;
-    hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), 0, st, P);
+    hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), 0, st, P, step_offset);
 }
 
 }  // namespace prism

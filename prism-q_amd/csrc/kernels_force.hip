@@ -1,7 +1,8 @@
 // kernels_force.hip -- force evaluation: nonbonded cluster-pair tiles,
-// bonded terms and exact 1-4 exceptions, all accumulated into int64 fixed
-// point (2^24 per kcal/mol/A) so the summed force does not depend on the
-// order in which tiles and terms retire.
+// bonded terms and exact 1-4 exceptions.  Every work item stores its partial
+// forces to its own location; a gather sums each atom's partials in a fixed
+// order into int64 fixed point (2^24 per kcal/mol/A), so the force is
+// bitwise reproducible and no device-scope atomics sit on the hot path.
 //
 // Replaces the FORCE_N / FORCE_NP1 gather of prism_amber_phase_body_v1
 // (nhs_amber_fused.cu:2981-3484) with its term functions compute_bond_force
@@ -70,7 +71,7 @@ __device__ inline void tile_pairs(const float4 xi, const float2 li, int n_act, i
 // quarter's current bounding box at the cutoff and the survivors compacted
 // into LDS.  The row is a full list (both directions), so each lane only
 // accumulates its own i-force; the four j-slot partials are summed at the end
-// of the unit and added once to the int64 fixed-point accumulator.
+// of the unit and stored as the unit's partial (no atomics).
 template <bool ENERGY>
 __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -112,32 +113,50 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
         const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
         const float hx = 0.5f * (hi[0] - lo[0]), hy = 0.5f * (hi[1] - lo[1]), hz = 0.5f * (hi[2] - lo[2]);
 
+        // the unit's tile indices and mask slots, one per lane, then a software
+        // pipeline: the next tile's j-atoms load while this tile computes
+        const int n_t = t1 - t0;
+        int my_jb = 0, my_ms = -1;
+        if (lane < n_t) {
+            my_jb = P.row_j[(size_t)b * MAX_ROW + t0 + lane];
+            my_ms = P.row_m[(size_t)b * MAX_ROW + t0 + lane];
+        }
+        int jb = __shfl(my_jb, 0);
+        float4 xj = P.xq[jb * BLOCK + lane];
+        float2 lj = P.lj[jb * BLOCK + lane];
         float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
-        for (int t = t0; t < t1; ++t) {
-            const int jb = P.row_j[(size_t)b * MAX_ROW + t];
-            const int ms = P.row_m[(size_t)b * MAX_ROW + t];
-            const float4 xj = P.xq[jb * BLOCK + lane];
-            const float2 lj = P.lj[jb * BLOCK + lane];
+        for (int k = 0; k < n_t; ++k) {
+            const int ms = __shfl(my_ms, k);
+            float4 xn = xj;
+            float2 ln = lj;
+            if (k + 1 < n_t) {
+                const int jn = __shfl(my_jb, k + 1);
+                xn = P.xq[jn * BLOCK + lane];
+                ln = P.lj[jn * BLOCK + lane];
+            }
             const float ex = fmaxf(fabsf(xj.x - cx) - hx, 0.0f);
             const float ey = fmaxf(fabsf(xj.y - cy) - hy, 0.0f);
             const float ez = fmaxf(fabsf(xj.z - cz) - hz, 0.0f);
             const bool active = ex * ex + ey * ey + ez * ez <= reach2;
             const unsigned long long m = __ballot(active);
             const int n_act = __popcll(m);
-            if (n_act == 0) continue;
-            if (active) {
-                const int pos = __popcll(m & lt);
-                jpos[pos] = xj;
-                jpar[pos] = make_float4(lj.x, lj.y, __int_as_float(lane), 0.0f);
+            if (n_act > 0) {
+                if (active) {
+                    const int pos = __popcll(m & lt);
+                    jpos[pos] = xj;
+                    jpar[pos] = make_float4(lj.x, lj.y, __int_as_float(lane), 0.0f);
+                }
+                wave_lds_fence();
+                if (ms >= 0) {
+                    const unsigned long long excl = P.masks[((size_t)b * MAX_MASKS + ms) * 64 + q * SUB + il];
+                    tile_pairs<ENERGY, true>(xi, li, n_act, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
+                } else {
+                    tile_pairs<ENERGY, false>(xi, li, n_act, js, jpos, jpar, 0ull, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
+                }
+                wave_lds_fence();
             }
-            wave_lds_fence();
-            if (ms >= 0) {
-                const unsigned long long excl = P.masks[((size_t)b * MAX_MASKS + ms) * 64 + q * SUB + il];
-                tile_pairs<ENERGY, true>(xi, li, n_act, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
-            } else {
-                tile_pairs<ENERGY, false>(xi, li, n_act, js, jpos, jpar, 0ull, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
-            }
-            wave_lds_fence();
+            xj = xn;
+            lj = ln;
         }
         // sum the four j-slot partials of each i-atom (lanes il, il+16, il+32,
         // il+48) in a fixed order and store the unit's partial; the gather
@@ -293,41 +312,65 @@ __global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
     if (ENERGY) P.term_e[tid] = energy;
 }
 
-// One thread per arena atom: the int64 fixed-point force is the sum, in unit
-// order, of the nonbonded partials of the atom's block quarter, plus the
-// atom's bonded contribution slots in slot order.
+__device__ inline long long wave_xor_ll(long long v, int m) {
+    const int lo = __shfl_xor((int)(v & 0xffffffffll), m), hi = __shfl_xor((int)(v >> 32), m);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// One wave per block quarter (16 sorted slots), four lanes per slot: lane
+// (k, il) converts the nonbonded unit partials c = k, k+4, ... and the bonded
+// contribution slots k, k+4, ... of slot il to int64 fixed point; the four
+// lanes' sums are then combined.  Integer addition is associative, so the
+// force is bitwise independent of this split.  For a given k the 16 lanes of a
+// quarter read 256 contiguous bytes of each unit partial.
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_gather(DevPtrs P, unsigned mask) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= P.n_atoms_arena) return;
-    const int s = P.atom_sys[a];
-    if (s < 0 || !P.sys[s].active) return;
+    const int lane = threadIdx.x & 63;
+    const int il = lane & (SUB - 1), k = lane >> 4;
+    const int quarter = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int slot = quarter * SUB + il;
+    if (slot >= P.n_atoms_arena) return;  // whole waves: n_atoms_arena is a multiple of 64
+    const int a = P.perm[slot];
+    const int s = a >= 0 ? P.atom_sys[a] : -1;
+    const bool live = s >= 0 && P.sys[s].active;
     long long acc[3] = {0, 0, 0};
     double e = 0.0;
-    if (mask & PRISM_MD_TERM_NONBONDED) {
-        const int slot = P.slot_of[a];
-        const int b = slot / BLOCK, q = (slot % BLOCK) / SUB, il = slot % SUB;
-        const int per_q = P.unit_count[b] / QUARTERS;
-        const float4 *uf = P.unit_f + ((size_t)P.unit_off[b] + (size_t)q * per_q) * SUB + il;
-        for (int c = 0; c < per_q; ++c) {
-            const float4 p = uf[(size_t)c * SUB];
-            acc[0] += to_fixed(p.x, FORCE_FIXED_BITS);
-            acc[1] += to_fixed(p.y, FORCE_FIXED_BITS);
-            acc[2] += to_fixed(p.z, FORCE_FIXED_BITS);
-            if (ENERGY) e += p.w;
+    if (live) {
+        if (mask & PRISM_MD_TERM_NONBONDED) {
+            const int b = slot / BLOCK, q = (slot % BLOCK) / SUB;
+            const int per_q = P.unit_count[b] / QUARTERS;
+            const float4 *uf = P.unit_f + ((size_t)P.unit_off[b] + (size_t)q * per_q) * SUB + il;
+#pragma unroll 2
+            for (int c = k; c < per_q; c += 4) {
+                const float4 p = uf[(size_t)c * SUB];
+                acc[0] += to_fixed(p.x, FORCE_FIXED_BITS);
+                acc[1] += to_fixed(p.y, FORCE_FIXED_BITS);
+                acc[2] += to_fixed(p.z, FORCE_FIXED_BITS);
+                if (ENERGY) e += p.w;
+            }
+        }
+        for (int j = P.inc_off[a] + k; j < P.inc_off[a + 1]; j += 4) {
+            const float *c = P.contrib + (size_t)j * 3;
+            acc[0] += to_fixed(c[0], FORCE_FIXED_BITS);
+            acc[1] += to_fixed(c[1], FORCE_FIXED_BITS);
+            acc[2] += to_fixed(c[2], FORCE_FIXED_BITS);
         }
     }
-    for (int k = P.inc_off[a]; k < P.inc_off[a + 1]; ++k) {
-        const float *c = P.contrib + (size_t)k * 3;
-        acc[0] += to_fixed(c[0], FORCE_FIXED_BITS);
-        acc[1] += to_fixed(c[1], FORCE_FIXED_BITS);
-        acc[2] += to_fixed(c[2], FORCE_FIXED_BITS);
+    for (int c = 0; c < 3; ++c) {
+        acc[c] += wave_xor_ll(acc[c], 16);
+        acc[c] += wave_xor_ll(acc[c], 32);
     }
-    long long *f = P.f + (size_t)a * 4;
-    f[0] = acc[0];
-    f[1] = acc[1];
-    f[2] = acc[2];
-    if (ENERGY) P.atom_e[a] = e;
+    if (ENERGY) {
+        e += __shfl_xor(e, 16);
+        e += __shfl_xor(e, 32);
+    }
+    if (live && k == 0) {
+        long long *f = P.f + (size_t)a * 4;
+        f[0] = acc[0];
+        f[1] = acc[1];
+        f[2] = acc[2];
+        if (ENERGY) P.atom_e[a] = e;
+    }
 }
 
 // One workgroup per system: FP64 sums in a fixed order of the per-atom
@@ -436,9 +479,9 @@ __global__ __launch_bounds__(256) void k_pairs(DevPtrs P, int mode, RefGrid g, i
 
 void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st) {
     // persistent grid: enough waves to cover the unit list of a typical row
-    // (80 tiles -> 40 units per block), capped at 16 waves per CU
+    // (80 tiles -> 40 units per block), capped at 32 waves per CU
     const int want = (P.n_blocks_arena * QUARTERS * 12 + NB_WAVES - 1) / NB_WAVES;
-    const int grid = max(1, min(want, 256 * 16 / NB_WAVES));
+    const int grid = max(1, min(want, 256 * 32 / NB_WAVES));
     if (energy) hipLaunchKernelGGL(k_nonbonded<true>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
     else hipLaunchKernelGGL(k_nonbonded<false>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
 }
@@ -452,7 +495,7 @@ void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_
 }
 
 void launch_gather(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t st) {
-    const int grid = (P.n_atoms_arena + 255) / 256;
+    const int grid = (P.n_atoms_arena / SUB * 64 + 255) / 256;  // one wave per block quarter
     if (energy) hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(256), 0, st, P, term_mask);
     else hipLaunchKernelGGL(k_gather<false>, dim3(grid), dim3(256), 0, st, P, term_mask);
 }

@@ -53,7 +53,7 @@ struct SysDesc {
     int com_every;
     int constraint_max_iter;
     int sort_bits;                  // max bits per axis of the Hilbert sort key
-    int pad0;
+    int resort_every;               // full re-sort on every Nth rebuild
     float dt;
     float cutoff, cutoff2, list_cutoff;
     float half_skin2;               // (skin/2)^2 displacement trigger
@@ -76,8 +76,8 @@ struct SysDesc {
 struct SysState {
     unsigned long long step;        // index of the step being executed
     long long rebuild_step;         // step whose positions the pair list was built at (-1: before step 0)
-    int rebuild_flag;               // set by the integrator or the host, consumed by the sort
-    int rows_dirty;                 // set by the sort for the row builder of the same rebuild
+    int rebuild_flag;               // set by the integrator or the host, consumed by the row builder
+    int resort_flag;                // set by the host: the next rebuild re-sorts (positions replaced)
     int status;                     // ST_* bits
     int n_rebuilds;
     double com_v[3];                // centre-of-mass velocity for the next COM step
@@ -99,6 +99,13 @@ enum : int {
     FAULT_RATTLE_4 = 5,     // final RATTLE
     FAULT_NONFINITE = 6,    // non-finite position or velocity
 };
+
+// A rebuild re-sorts the atoms into blocks when the host asked for it or on
+// every resort_every-th rebuild; the other rebuilds keep the blocks and only
+// refresh their boxes and the rows.
+__host__ __device__ inline bool resort_due(const SysDesc &D, const SysState &S) {
+    return S.resort_flag || D.resort_every <= 1 || S.n_rebuilds % D.resort_every == 0;
+}
 
 // Global engine counters (device-written).
 struct EngineCounters {
