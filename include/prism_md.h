@@ -175,10 +175,12 @@ typedef struct PrismMdStatsV1 {
 
 typedef struct PrismMdHandle PrismMdHandle;
 
-/* Fills *params with the defaults: dt 0.004 ps, 300 K, gamma 1/ps, cutoff 10 A,
- * skin 2 A, rebuild every 20 steps, COM removal every 100 steps, seed
- * 0x50524953, 100 constraint iterations, shake_tol 1e-10, rattle_tol 1e-12,
- * constant temperature and friction.
+/* Fills *params with the defaults: dt 0.004 ps (HMR), 300 K, gamma_base
+ * 10/ps (fused_engine.rs:13314) with the equilibration friction ramp from
+ * 1000/ps over the first 10000 steps (fused_engine.rs:15370-15378), cutoff
+ * 10 A, skin 2 A, rebuild every 20 steps, re-sort every 8th rebuild, COM
+ * removal every 100 steps, seed 0x50524953, 100 constraint iterations,
+ * shake_tol 1e-10, rattle_tol 1e-12, constant temperature.
  * Replaces the constructor defaults of fused_engine.rs:13314-13366. */
 int prism_md_default_params(PrismMdParamsV1 *params);
 
@@ -275,6 +277,15 @@ int prism_md_count_pairs(PrismMdHandle *h, uint64_t *n_pairs);
  * stream at the current positions and report the mean launch duration in ms,
  * measured with HIP events on that stream. */
 int prism_md_time_nonbonded(PrismMdHandle *h, int iters, double *mean_ms);
+
+/* Sample the nonbonded kernel inside prism_md_step: with every > 0, HIP
+ * events on the handle's stream bracket the nonbonded launch of every
+ * every-th step from now on (at most 1024 samples per prism_md_step call).
+ * Each call returns the mean duration (ms) and count of the samples taken
+ * since sampling was last (re)started; every = 0 stops sampling, and a call
+ * with a changed interval restarts the count.  No reference counterpart (the
+ * reference's timings are nsys receipts outside its tree, README.md:105-107). */
+int prism_md_kernel_timing(PrismMdHandle *h, int every, double *mean_ms, uint64_t *samples);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,7 @@ int Engine::d2h(void *dst, const void *src, size_t bytes) {
 
 Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
+    for (hipEvent_t e : ev_pool_) hipEventDestroy(e);
     if (graph_) hipGraphExecDestroy(graph_);
     for (void *p : allocs_) hipFree(p);
     if (own_stream_ && stream_) hipStreamDestroy(stream_);
@@ -146,6 +147,10 @@ static int validate(const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p, std::s
 }
 
 int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
+    for (const auto &sp : systems) {
+        std::string why;
+        if (validate(sp.topo, sp.params, why) != PRISM_MD_OK) return fail(PRISM_MD_E_INVALID, "invalid topology/params: " + why);
+    }
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(PRISM_MD_E_NODEVICE, "no HIP device");
     if (stream) {
@@ -153,10 +158,6 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
     } else {
         if (int rc = hipcheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream")) return rc;
         own_stream_ = true;
-    }
-    for (const auto &sp : systems) {
-        std::string why;
-        if (validate(sp.topo, sp.params, why) != PRISM_MD_OK) return fail(PRISM_MD_E_INVALID, "invalid topology/params: " + why);
     }
     // ---- arena layout
     int A = 0, W = 0;
@@ -493,7 +494,11 @@ void Engine::launch_step(const float *noise_n, const float *noise_np1) {
     launch_first_half(P_, noise_n, stream_);
     launch_sort(P_, stream_);
     launch_rows(P_, 0, stream_);
+    const bool sample = timing_every_ > 0 && timing_counter_++ % (uint64_t)timing_every_ == 0 &&
+                        2 * ev_used_ + 1 < ev_pool_.size();
+    if (sample) hipEventRecord(ev_pool_[2 * ev_used_], stream_);
     launch_nonbonded(P_, false, stream_);
+    if (sample) hipEventRecord(ev_pool_[2 * ev_used_++ + 1], stream_);
     launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
     launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
     launch_second_half(P_, noise_np1, stream_);
@@ -513,6 +518,10 @@ int Engine::check_status() {
         const std::string where = " (system " + std::to_string(s) + ", " + kinds[fk] + ", atom " +
                                   std::to_string(st[s].fault_atom - sys_[s].atom_begin) + ", step " +
                                   std::to_string(st[s].fault_step) + ")";
+        // the first fault decides the class (later steps may turn a failed
+        // constraint into non-finite values)
+        if ((b & ST_CONSTRAINT) && fk >= FAULT_RATTLE_1 && fk <= FAULT_RATTLE_4)
+            return fail(PRISM_MD_E_CONSTRAINT, "constraint iteration did not converge" + where);
         if (b & ST_NONFINITE) return fail(PRISM_MD_E_NONFINITE, "non-finite state" + where);
         if (b & ST_CONSTRAINT) return fail(PRISM_MD_E_CONSTRAINT, "constraint iteration did not converge" + where);
         if (b & ST_EXCEPTION_COINCIDENT) return fail(PRISM_MD_E_NONFINITE, "coincident exception atoms" + where);
@@ -637,7 +646,41 @@ int Engine::step(uint64_t n_steps, const float *noise) {
     }
     host_steps_ += n_steps;
     if (int rc = hipcheck(hipGetLastError(), "step launch")) return rc;
-    return check_status();
+    const int rc = check_status();
+    harvest_timing();
+    return rc;
+}
+
+// Durations of the sampled nonbonded launches of the last step() call (the
+// stream is synchronised by then).
+void Engine::harvest_timing() {
+    for (size_t k = 0; k < ev_used_; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev_pool_[2 * k], ev_pool_[2 * k + 1]) == hipSuccess) {
+            timed_ms_ += ms;
+            ++timed_n_;
+        }
+    }
+    ev_used_ = 0;
+}
+
+int Engine::kernel_timing(int every, double *mean_ms, uint64_t *samples) {
+    if (every < 0) return fail(PRISM_MD_E_INVALID, "timing interval");
+    harvest_timing();
+    if (mean_ms) *mean_ms = timed_n_ ? timed_ms_ / (double)timed_n_ : 0.0;
+    if (samples) *samples = timed_n_;
+    if (every != timing_every_ || every == 0) {
+        timed_ms_ = 0.0;
+        timed_n_ = 0;
+        timing_counter_ = 0;
+    }
+    if (every > 0 && ev_pool_.empty()) {
+        ev_pool_.resize(2048);
+        for (auto &e : ev_pool_)
+            if (int rc = hipcheck(hipEventCreate(&e), "timing event")) return rc;
+    }
+    timing_every_ = every;
+    return PRISM_MD_OK;
 }
 
 int Engine::step_graph(uint64_t n_steps, int graph_steps) {
@@ -933,6 +976,10 @@ int prism_md_count_pairs(PrismMdHandle *h, uint64_t *n) { return h && n ? h->eng
 int prism_md_minimize(PrismMdHandle *h, int max_iter, double f_tol, double *e_final, double *f_max) {
     return h ? h->eng.minimize(max_iter, f_tol, e_final, f_max) : PRISM_MD_E_INVALID;
 }
+int prism_md_kernel_timing(PrismMdHandle *h, int every, double *mean_ms, uint64_t *samples) {
+    return h ? h->eng.kernel_timing(every, mean_ms, samples) : PRISM_MD_E_INVALID;
+}
+
 int prism_md_time_nonbonded(PrismMdHandle *h, int iters, double *mean_ms) {
     return h && mean_ms && iters > 0 ? h->eng.time_nonbonded(iters, mean_ms) : PRISM_MD_E_INVALID;
 }

@@ -43,6 +43,9 @@ class Engine {
     int get_pairs(int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out);
     int count_pairs(int s, uint64_t *n);
     int time_nonbonded(int iters, double *mean_ms);
+    // sample the nonbonded launch of every `every`-th step with HIP events
+    // (0 disables); returns the mean duration over the samples so far
+    int kernel_timing(int every, double *mean_ms, uint64_t *samples);
     int minimize(int max_iter, double f_tol, double *e_final, double *f_max);
     int stats(int s, PrismMdStatsV1 *out);
     int set_active(int s, bool active);
@@ -62,6 +65,7 @@ class Engine {
     int refresh_all();            // rebuild lists + forces at current positions
     void launch_step(const float *noise_n, const float *noise_np1);
     void full_force();
+    void harvest_timing();
 
     struct HostSys {
         int atom_begin, n_atoms, n_slots;
@@ -78,6 +82,11 @@ class Engine {
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
     std::vector<void *> allocs_;
+    int timing_every_ = 0;        // sampled nonbonded timing (kernel_timing)
+    std::vector<hipEvent_t> ev_pool_;  // pairs: [2k] before, [2k+1] after
+    size_t ev_used_ = 0;
+    uint64_t timing_counter_ = 0, timed_n_ = 0;
+    double timed_ms_ = 0.0;
     bool alloc_failed_ = false;
     std::string err_;
     // host-side pair list of the last build_pairs

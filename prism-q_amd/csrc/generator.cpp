@@ -517,31 +517,39 @@ static int generate(PrismGenHandle &g) {
         for (int h = 0; h < n_h[k]; ++h) mass[first_h[k] + h] = 1.008;
     }
     // The sampled values are exactly the stated uniform draws -- N values of
-    // sigma ~ U(sigma_min, sigma_max), |q| ~ U(0, charge_max) with a random
-    // sign, epsilon ~ U(epsilon_min, epsilon_max) -- but they are handed out
-    // by rank: hydrogens receive the smaller draws, heavy atoms the larger,
-    // and |q| follows sigma's rank (one uniform u per atom).  Drawn per atom
-    // independently, a sigma = 1 A atom next to an opposite 0.8 e charge
-    // balances Coulomb against the r^-12 wall at ~0.7 A, and 3.5 A hydrogens
-    // pack at a reduced density of ~1.5; both give contact vibrations with
-    // periods of ~6-13 fs for 3 amu hydrogens, which a 4 fs integrator heats
-    // until SHAKE fails.
+    // sigma ~ U(sigma_min, sigma_max), q ~ U(-charge_max, charge_max),
+    // epsilon ~ U(epsilon_min, epsilon_max) -- but they are handed out by
+    // rank, as a force field would: hydrogens receive the smaller sigma and
+    // epsilon draws and the more positive charges, heavy atoms the larger
+    // sigma and epsilon and the more negative charges, and within each class
+    // |q| grows with sigma.  Drawn per atom independently, oppositely charged
+    // small hydrogens pair up against the r^-12 wall (a sigma = 1 A atom next
+    // to an opposite 0.8 e charge balances at ~0.7 A; 2.2 A hydrogens with
+    // +-0.4 e meet at 1.8 A), and those 3 amu contacts vibrate with periods of
+    // a few femtoseconds, which a 4 fs integrator heats until SHAKE fails.
     std::vector<int> order(N);
     for (int a = 0; a < N; ++a) order[a] = a;
     std::vector<double> key(N);
     for (int a = 0; a < N; ++a) key[a] = (mass[a] < 2.0 ? 0.0 : 1.0) + rng.uniform();  // H first, random within
     std::sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
-    std::vector<double> us(N), ue(N);
+    int n_hyd = 0;
+    for (int a = 0; a < N; ++a) n_hyd += mass[a] < 2.0;
+    std::vector<double> us(N), ue(N), uq(N);
     for (int a = 0; a < N; ++a) {
         us[a] = rng.uniform();
         ue[a] = rng.uniform();
+        uq[a] = P.charge_max * (2.0 * rng.uniform() - 1.0);
     }
     std::sort(us.begin(), us.end());
     std::sort(ue.begin(), ue.end());
+    std::sort(uq.begin(), uq.end(), [](double x, double y) { return x > y; });  // most positive first
+    // hydrogens: the n_hyd most positive charges, ascending with sigma rank;
+    // heavy atoms: the rest, descending (more negative) with sigma rank
+    std::sort(uq.begin(), uq.begin() + n_hyd);
     double qsum = 0.0;
     for (int r = 0; r < N; ++r) {
         const int a = order[r];
-        q[a] = (rng.below(2) ? 1.0 : -1.0) * P.charge_max * us[r];
+        q[a] = uq[r];
         qsum += q[a];
         g.sigma[a] = (float)(P.sigma_min + (P.sigma_max - P.sigma_min) * us[r]);
         g.epsilon[a] = (float)(P.epsilon_min + (P.epsilon_max - P.epsilon_min) * ue[r]);

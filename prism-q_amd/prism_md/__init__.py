@@ -131,6 +131,7 @@ def md_lib():
             "prism_md_last_error": ([H], C.c_char_p),
             "prism_md_count_pairs": ([H, C.POINTER(C.c_uint64)], C.c_int),
             "prism_md_time_nonbonded": ([H, C.c_int, C.POINTER(C.c_double)], C.c_int),
+            "prism_md_kernel_timing": ([H, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)], C.c_int),
             "prism_md_minimize": ([H, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         }
         for name, (args, res) in sig.items():
@@ -375,6 +376,13 @@ class FusedEngine:
         n = C.c_uint64()
         self._check(self.lib.prism_md_count_pairs(self.h, C.byref(n)), "count_pairs")
         return int(n.value)
+
+    def kernel_timing(self, every: int):
+        """(Re)start sampling the nonbonded launch of every `every`-th step
+        (0 stops); returns (mean ms, samples) of the samples so far."""
+        ms, n = C.c_double(), C.c_uint64()
+        self._check(self.lib.prism_md_kernel_timing(self.h, every, C.byref(ms), C.byref(n)), "kernel_timing")
+        return float(ms.value), int(n.value)
 
     def time_nonbonded(self, iters: int = 20) -> float:
         ms = C.c_double()

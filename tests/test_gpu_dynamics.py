@@ -1,0 +1,138 @@
+"""Dynamics and error paths on the GPU: thermostat temperature, NVE energy
+drift, state round trips, capacity and constraint errors reported as error
+codes (forced from the host, so every kernel exits normally)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_state_round_trip(pm):
+    top = pm.Topology.generate(2000, seed=4)
+    with pm.FusedEngine(top) as eng:
+        x = eng.get_positions()
+        assert np.array_equal(x, top["positions"].reshape(-1, 3))
+        v = np.random.default_rng(0).normal(0, 1, x.shape).astype(np.float32)
+        x2 = x + np.float32(0.01)
+        eng.set_state(x2, v)
+        assert np.array_equal(eng.get_positions(), x2)
+        assert np.array_equal(eng.get_velocities(), v)
+
+
+def test_langevin_temperature(pm):
+    top = pm.Topology.generate(2000, seed=4)
+    p = pm.default_params(temperature_K=320.0, end_temperature_K=320.0, equilibration_steps=2000)
+    with pm.FusedEngine(top, p) as eng:
+        eng.minimize(1000, 1.0)
+        eng.step(1500)
+        temps = []
+        for _ in range(10):
+            eng.step(100)
+            temps.append(eng.energies()["temperature"])
+        assert abs(np.mean(temps) - 320.0) < 15.0, temps
+        assert eng.stats()["status"] == 0
+
+
+def _nve_series(pm, top, x, v, dt, chunks, chunk_ps):
+    p = pm.default_params(dt_ps=dt, gamma_ps=0.0, equilibration_steps=0, temperature_K=100.0,
+                          end_temperature_K=100.0, com_remove_every=0)
+    with pm.FusedEngine(top.with_positions(x), p) as eng:
+        eng.set_state(v=v)
+        eng.step(1)  # RATTLE projects the Maxwell-Boltzmann velocities
+        d = eng.energies()
+        e0 = d["potential"] + d["kinetic"]
+        out = []
+        for _ in range(chunks):
+            eng.step(int(round(chunk_ps / dt)))
+            d = eng.energies()
+            out.append(d["potential"] + d["kinetic"] - e0)
+        return e0, np.array(out), d["kinetic"]
+
+
+@pytest.fixture(scope="module")
+def nve_start(pm):
+    top = pm.Topology.generate(2000, seed=4)
+    with pm.FusedEngine(top) as eng:
+        eng.minimize(5000, 0.1)
+        x = eng.get_positions()
+    return top, x, pm.maxwell_boltzmann(top["masses"], 100.0, 3)
+
+
+def test_nve_energy_conservation(pm, nve_start):
+    """gamma = 0 selects exact kicks (no force clamp).  At 0.5 fs the total
+    energy of the relaxed 2,000-atom chain stays within 0.5 % of the kinetic
+    energy over 5 ps; the error grows with dt (integration error, not a leak)."""
+    top, x, v = nve_start
+    _, fine, ke = _nve_series(pm, top, x, v, 0.0005, 10, 0.5)
+    _, coarse, _ = _nve_series(pm, top, x, v, 0.001, 10, 0.5)
+    print(f"[nve] 0.5 fs: {np.round(fine, 2).tolist()}  1 fs: {np.round(coarse, 2).tolist()}  KE {ke:.0f}",
+          flush=True)
+    assert np.abs(fine).max() < 0.005 * ke
+    assert abs(coarse[-1]) > 2.0 * abs(fine[-1])
+
+
+def test_nve_drift_matches_oracle(pm, oracle, nve_start):
+    """The 2 fs energy drift of this model is the algorithm's, not the GPU's:
+    the FP64 oracle from the same state drifts alike (within 35 % or
+    3 kcal/mol at 0.5 and 1 ps; the trajectories separate chaotically)."""
+    top, x, v = nve_start
+    _, g, _ = _nve_series(pm, top, x, v, 0.002, 2, 0.5)
+    p = pm.default_params(dt_ps=0.002, gamma_ps=0.0, equilibration_steps=0, temperature_K=100.0,
+                          end_temperature_K=100.0, com_remove_every=0)
+    t = top.with_positions(x)
+    xo, vo, _ = oracle.run(t, p, x.astype(np.float64), v.astype(np.float64), 0, 1)
+    e0 = sum(oracle.energies(t, xo, vo))
+    o = []
+    for _ in range(2):
+        xo, vo, _ = oracle.run(t, p, xo, vo, 0, 250)
+        o.append(sum(oracle.energies(t, xo, vo)) - e0)
+    print(f"[nve 2 fs] gpu {np.round(g, 2).tolist()} oracle {np.round(o, 2).tolist()}", flush=True)
+    for a, b in zip(g, o):
+        assert abs(a - b) <= max(3.0, 0.35 * abs(b))
+
+
+def test_reference_grid_capacity_is_an_error(pm):
+    x = np.zeros((64, 3), np.float32)
+    x[:, 0] = np.arange(64) * 6.0  # 378 A along x: more than 32 reference cells
+    import sys
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    from toy import make_topology
+    top = make_topology(pm, x)
+    with pm.FusedEngine(top) as eng:
+        with pytest.raises(pm.PrismMdError) as ex:
+            eng.pairs(0)
+        assert ex.value.rc == pm.E_CAPACITY
+        assert len(eng.pairs(1)) == 63 + 62  # r <= 12 A neighbours along the line
+
+
+def test_constraint_failure_is_an_error(pm):
+    """One SHAKE/RATTLE sweep cannot meet a 1e-14 tolerance after a hot step:
+    the step returns E_CONSTRAINT with the failing atom, no trap."""
+    top = pm.Topology.generate(2000, seed=4)
+    p = pm.default_params(constraint_max_iter=1, shake_tol=1e-14, rattle_tol=1e-14)
+    with pm.FusedEngine(top, p) as eng:
+        eng.set_state(v=pm.maxwell_boltzmann(top["masses"], 600.0, 5))
+        with pytest.raises(pm.PrismMdError) as ex:
+            eng.step(5)
+        assert ex.value.rc == pm.E_CONSTRAINT
+        assert "atom" in str(ex.value)
+
+
+def test_minimize_lowers_energy(pm):
+    top = pm.Topology.generate(2000, seed=8)
+    with pm.FusedEngine(top) as eng:
+        e0 = eng.energies()["potential"]
+        e1, fmax = eng.minimize(1000, 1.0)
+        assert e1 < e0 and fmax < 50.0
+        x = eng.get_positions()
+        c = top["cluster_central"]
+        h = top["cluster_hydrogens"].reshape(-1, 3)
+        d = top["cluster_bond_lengths"].reshape(-1, 3)
+        for s in range(3):
+            ok = h[:, s] >= 0
+            if not ok.any():
+                continue
+            r = np.linalg.norm(x[h[ok, s]] - x[c[ok]], axis=1)
+            assert np.abs(r - d[ok, s]).max() < 1e-4

@@ -1,0 +1,127 @@
+"""The oracle's pair lists, pinned by independent numpy restatements:
+
+- brute mode: every pair with FP32 r2 <= 144.0f, r2 = (dx*dx + dy*dy) + dz*dz
+  rounded after each operation (numpy float32 does not contract), dx = xj - xi,
+  minus 1-2/1-3 exclusions (exclusions that are not 1-4 exception partners);
+- reference mode: the cell scheme of fused_engine.rs:12285-12330 and
+  nhs_amber_fused.cu:11690-11728 (10 A cells, origin = bbox min - 10 A,
+  n = ceil((extent + 20)/10), truncating assignment with clamp, 27 cells);
+- both modes agree on every pair closer than 10 A (REVIEW_BUNDLE.md:3284);
+- the reference's capacities (32 cells per axis, 512 atoms per cell, 1024
+  neighbours per atom) are errors, not silent truncation.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from toy import make_topology
+
+
+def list_exclusions(top):
+    """Skip set of the pair list: exclusions minus exception partners."""
+    off, idx = top["exclusion_offsets"], top["exclusion_atoms"]
+    exc = top["exception_atoms"].reshape(-1, 2)
+    ex14 = {(min(a, b), max(a, b)) for a, b in exc}
+    out = set()
+    for i in range(top.n_atoms):
+        for j in idx[off[i]:off[i + 1]]:
+            p = (min(i, int(j)), max(i, int(j)))
+            if p not in ex14:
+                out.add(p)
+    return out
+
+
+def r2_f32(x):
+    x = x.astype(np.float32)
+    dx = x[None, :, 0] - x[:, None, 0]
+    dy = x[None, :, 1] - x[:, None, 1]
+    dz = x[None, :, 2] - x[:, None, 2]
+    return (dx * dx + dy * dy) + dz * dz  # float32, one rounding per operation
+
+
+def numpy_pairs(top, x, xg, mode):
+    n = top.n_atoms
+    r2 = r2_f32(x)
+    keep = np.triu(~(r2 > np.float32(144.0)), 1)
+    if mode == 0:
+        xg = xg.astype(np.float32)
+        lo, hi = xg.min(axis=0), xg.max(axis=0)
+        origin = lo - np.float32(10.0)
+        ext = (hi - lo) + np.float32(20.0)
+        dims = np.maximum(1, np.ceil(ext / np.float32(10.0))).astype(np.int64)
+        cell = ((x.astype(np.float32) - origin) * np.float32(0.1)).astype(np.int64)
+        cell = np.clip(cell, 0, dims - 1)
+        near = np.ones((n, n), bool)
+        for c in range(3):
+            near &= np.abs(cell[:, None, c] - cell[None, :, c]) <= 1
+        keep &= near
+    i, j = np.nonzero(keep)
+    ex = list_exclusions(top)
+    pairs = [(a, b) for a, b in zip(i.tolist(), j.tolist()) if (a, b) not in ex]
+    return np.array(sorted(pairs), np.int32).reshape(-1, 2)
+
+
+@pytest.fixture(scope="module")
+def chain2k(pm):
+    return pm.Topology.generate(2000, seed=3)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pairs_match_numpy(oracle, chain2k, mode):
+    x = chain2k["positions"].reshape(-1, 3)
+    got = oracle.pairs(chain2k, x, x, mode)
+    want = numpy_pairs(chain2k, x, x, mode)
+    assert got.shape == want.shape and np.array_equal(got, want)
+
+
+def test_pairs_after_motion(oracle, chain2k, rng):
+    """Grid from creation positions, pairs at displaced positions (the
+    reference keeps its grid from construction)."""
+    x0 = chain2k["positions"].reshape(-1, 3)
+    x = (x0 + rng.normal(0, 0.8, x0.shape)).astype(np.float32)
+    for mode in (0, 1):
+        assert np.array_equal(oracle.pairs(chain2k, x, x0, mode), numpy_pairs(chain2k, x, x0, mode))
+
+
+def test_modes_agree_inside_cutoff(oracle, chain2k):
+    x = chain2k["positions"].reshape(-1, 3)
+    ref, brute = oracle.pairs(chain2k, x, x, 0), oracle.pairs(chain2k, x, x, 1)
+    sref = set(map(tuple, ref.tolist()))
+    sbr = set(map(tuple, brute.tolist()))
+    assert sref <= sbr
+    d = np.linalg.norm(x[brute[:, 0]] - x[brute[:, 1]], axis=1)
+    inner = {p for p, r in zip(map(tuple, brute.tolist()), d) if r < 10.0}
+    assert inner <= sref
+
+
+def test_one_four_partners_kept_and_one_two_dropped(oracle, small_top):
+    x = small_top["positions"].reshape(-1, 3)
+    s = set(map(tuple, oracle.pairs(small_top, x, x, 1).tolist()))
+    b = small_top["bond_atoms"].reshape(-1, 2)
+    assert not any((min(i, j), max(i, j)) in s for i, j in b)
+    e = small_top["exception_atoms"].reshape(-1, 2)
+    assert all((min(i, j), max(i, j)) in s for i, j in e)
+
+
+def test_capacity_too_many_cells(pm, oracle):
+    x = np.zeros((40, 3), np.float32)
+    x[:, 0] = np.arange(40) * 9.0  # extent 351 A -> 38 cells on x
+    t = make_topology(pm, x)
+    assert oracle.pairs(t, x, x, 0) == -2
+    assert oracle.pairs(t, x, x, 1).shape[0] == 39  # brute mode has no grid
+
+
+def test_capacity_crowded_cell(pm, oracle, rng):
+    x = rng.uniform(0, 5, (600, 3)).astype(np.float32)  # 600 atoms in one 10 A cell
+    t = make_topology(pm, x)
+    assert oracle.pairs(t, x, x, 0) == -2
+
+
+def test_capacity_neighbour_row(pm, oracle, rng):
+    # 1100 atoms in a 13 A cube centred on a cell corner: <= 512 per cell,
+    # but central atoms see more than 1024 neighbours within 12 A
+    x = rng.uniform(-6.5, 6.5, (1100, 3)).astype(np.float32)
+    x += np.float32(20.0)  # grid origin = min - 10 puts a corner near the centre
+    t = make_topology(pm, x)
+    assert oracle.pairs(t, x, x, 0) == -2
