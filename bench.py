@@ -64,6 +64,24 @@ def cpu_baseline(pm, top, x, v, params, target_s):
                       f"{sps:.3f} steps/s, OpenMP threads={threads}"}
 
 
+def copy_calibration(mib: int, reps: int = 5):
+    """Device-to-device copies of a known size through the HIP runtime, so a
+    counter pass can scale FETCH_SIZE / WRITE_SIZE: each copy moves exactly
+    mib MiB in and mib MiB out."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {reps} device-to-device copies of {mib} MiB")
+
+
 def traffic_per_launch():
     path = os.path.join(ROOT, "profiles", "nonbonded_25k_traffic.json")
     if not os.path.exists(path):
@@ -80,6 +98,8 @@ def main():
     ap.add_argument("--atoms", type=int, default=25000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--sample-every", type=int, default=10, help="nonbonded timing sample interval (steps)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run device-to-device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported; this build runs "
@@ -89,6 +109,8 @@ def main():
     import numpy as np
     import prism_md as pm
 
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
     t0 = time.perf_counter()
     top = pm.Topology.generate(args.atoms, seed=GEN_SEED)
     params = pm.default_params()

@@ -3,26 +3,36 @@
 // TEST INFRASTRUCTURE ONLY: the parity checker.  Loaded by tests/, the
 // smoke check and the benchmark's CPU-baseline leg; never by product code.
 //
-// What pins each function (beyond being a term-by-term restatement):
+// What pins each function (beyond being a term-by-term restatement); the
+// checks live in tests/test_oracle_*.py:
 //   oracle_forces  bond/angle/dihedral/nonbonded/exception: central finite
-//                  differences of the oracle's own energies (tests), closed
-//                  forms (2-atom bond, 3-atom angle at theta0, planar torsions,
-//                  LJ minimum at 2^(1/6) sigma, V(rc) = 0 after the shift),
-//                  Newton's third law and sum F = 0, and an independent numpy
-//                  energy + numerical-gradient evaluation (tests/golden).
-//   oracle_pairs   brute force vs cell scheme on random clouds, the stencil
-//                  defect of the 10 A cell scheme for 10-12 A pairs
-//                  (scripts/physics_refs/neighbor_list.py:1-11) planted on the
-//                  CPU, agreement of both modes for all pairs within 10 A.
-//   oracle_run     closed forms: free particle drift, OU stationary variance
-//                  kB T/m*418.4, the 0.5*dt*418.4 half-kick constant
-//                  (fused_engine.rs:4723-4731), X-H distance after SHAKE,
-//                  zero bond-parallel relative velocity after RATTLE, NVE
-//                  energy conservation at gamma = 0.
+//                  differences of the oracle's own energies, closed forms
+//                  (2-atom bond, angle at and off theta0, torsion at phi = 0,
+//                  pi/2, pi, LJ minimum at 2^(1/6) sigma, Coulomb pair,
+//                  V(rc) = 0 after the shift, exceptions without cutoff),
+//                  sum F = 0 and sum r x F = 0 per term, and a planted sign
+//                  error that the finite-difference check must reject.
+//   oracle_pairs   an independent numpy brute force and cell scheme (float32,
+//                  one rounding per operation), the known stencil defect of
+//                  10 A cells (only 10-12 A pairs missed,
+//                  scripts/physics_refs/neighbor_list.py:1-11), agreement of
+//                  both modes on every pair within 10 A, and the reference's
+//                  capacity errors.
+//   oracle_run     closed forms: OU half-steps with explicit noise, the OU
+//                  stationary variance 418.4 kB T/m, free-particle drift,
+//                  velocity Verlet with the 0.5*dt*418.4 half-kick constant
+//                  (fused_engine.rs:4723-4731), X-H lengths after SHAKE, zero
+//                  bond-parallel relative velocity after RATTLE, momentum
+//                  conservation of SHAKE, second-order NVE energy error.
 //   oracle_philox  Random123 known-answer vectors quoted at
-//                  scripts/physics_refs/rng_streams.py:82-88.
+//                  scripts/physics_refs/rng_streams.py:82-88, and the keyed
+//                  counter layout of rng_streams.py:75-79.
 //   oracle_schedule the director table of
-//                  crates/prism-nhs/tests/protocol_director_thermal_contract.cpp:30-66.
+//                  crates/prism-nhs/tests/protocol_director_thermal_contract.cpp:13-66
+//                  and the friction ramp of fused_engine.rs:15370-15378.
+// Parity unpinned: the reference's own Langevin noise (cuRAND XORWOW, a
+// library outside the reference tree) is not reproduced; integrator parity
+// is checked with the noise supplied as an explicit array.
 #include "oracle.h"
 
 #include <algorithm>

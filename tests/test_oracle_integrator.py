@@ -38,6 +38,22 @@ def test_ou_and_drift_closed_form(pm, oracle, rng):
     assert np.allclose(v1, c1 * va + c2 * nz[0, 1].astype(np.float64), rtol=0, atol=1e-12)
 
 
+def test_ou_stationary_variance(pm, oracle):
+    # a force-free atom under the Philox-driven thermostat samples
+    # v ~ N(0, 418.4 kB T / m) per axis; samples 50 steps (> 4/gamma) apart
+    m, T = 12.0, 300.0
+    t = make_topology(pm, [[0, 0, 0]], masses=[m])
+    p = pm.default_params(temperature_K=T, end_temperature_K=T, gamma_ps=20.0, equilibration_steps=0,
+                          com_remove_every=0)
+    x, v = np.zeros((1, 3)), np.zeros((1, 3))
+    samples = []
+    for k in range(400):
+        x, v, _ = oracle.run(t, p, x, v, 50 * k, 50)
+        samples.append(v[0].copy())
+    var = np.var(np.array(samples))
+    assert var == pytest.approx(CONV * KB * T / m, rel=0.1)
+
+
 def test_free_particle_nve(pm, oracle):
     t = make_topology(pm, [[0, 0, 0]], masses=[3.0])
     p = pm.default_params(gamma_ps=0.0, equilibration_steps=0)

@@ -93,6 +93,12 @@ def test_modes_agree_inside_cutoff(oracle, chain2k):
     d = np.linalg.norm(x[brute[:, 0]] - x[brute[:, 1]], axis=1)
     inner = {p for p, r in zip(map(tuple, brute.tolist()), d) if r < 10.0}
     assert inner <= sref
+    # the cell scheme's known defect (scripts/physics_refs/neighbor_list.py:1-11):
+    # 10 A cells with a 27-cell stencil miss only pairs between 10 and 12 A
+    missed = sbr - sref
+    assert missed, "expected some 10-12 A pairs outside the 27-cell stencil"
+    dm = {p: r for p, r in zip(map(tuple, brute.tolist()), d)}
+    assert all(dm[p] >= 10.0 for p in missed)
 
 
 def test_one_four_partners_kept_and_one_two_dropped(oracle, small_top):
@@ -113,15 +119,21 @@ def test_capacity_too_many_cells(pm, oracle):
 
 
 def test_capacity_crowded_cell(pm, oracle, rng):
-    x = rng.uniform(0, 5, (600, 3)).astype(np.float32)  # 600 atoms in one 10 A cell
+    x = rng.uniform(0, 5, (600, 3)).astype(np.float32)  # 600 atoms (< 1024 neighbours) in one 10 A cell
     t = make_topology(pm, x)
     assert oracle.pairs(t, x, x, 0) == -2
 
 
 def test_capacity_neighbour_row(pm, oracle, rng):
-    # 1100 atoms in a 13 A cube centred on a cell corner: <= 512 per cell,
-    # but central atoms see more than 1024 neighbours within 12 A
-    x = rng.uniform(-6.5, 6.5, (1100, 3)).astype(np.float32)
-    x += np.float32(20.0)  # grid origin = min - 10 puts a corner near the centre
+    # 1100 atoms in an 8 A cube split by a cell corner into 8 cells of ~140
+    # (< 512 each), every atom near the middle has > 1024 neighbours within 12 A
+    c = np.float32(50.0)
+    cube = rng.uniform(c - 4, c + 4, (1100, 3)).astype(np.float32)
+    anchor = np.full((1, 3), c - 20, np.float32)  # grid min c-20 -> origin c-30: c is a cell corner
+    x = np.concatenate([cube, anchor])
     t = make_topology(pm, x)
+    cells = ((x[:-1] - (c - 30)) * np.float32(0.1)).astype(int)
+    _, counts = np.unique(cells, axis=0, return_counts=True)
+    assert counts.max() <= 512 and len(counts) == 8
     assert oracle.pairs(t, x, x, 0) == -2
+    assert oracle.pairs(t, x, x, 1).shape[0] > 0  # brute mode has no row capacity
