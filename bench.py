@@ -18,7 +18,11 @@ One JSON line on stdout:
                (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, scaled by a
                copy-kernel calibration), else null;
   cpu_baseline the FP64 oracle (C++/OpenMP restatement of the reference step)
-               on a bounded sample of the same workload on the host cores.
+               on a bounded sample of the same workload on the host cores;
+  ensemble     256 systems of log-uniform size in [500, 20000] atoms at
+               280-340 K in one batched engine stepped through a captured
+               hipGraph: aggregate atom-steps/s, p50/p99 per-step latency,
+               and the same systems run one at a time on the plain path.
 """
 from __future__ import annotations
 
@@ -82,6 +86,61 @@ def copy_calibration(mib: int, reps: int = 5):
     log(f"copy calibration: {reps} device-to-device copies of {mib} MiB")
 
 
+def ensemble_leg(pm, n_systems, steps, graph_steps, seed=0x50524953 + 3):
+    """Batched ensemble (one launch per phase over all systems, captured
+    hipGraph) against the same systems run one at a time on the plain path.
+    Sizes log-uniform in [500, 20000] atoms, temperatures uniform in
+    [280, 340] K, 4 fs, per-system generator and noise seeds."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(500), np.log(20000), n_systems)).astype(int)
+    temps = rng.uniform(280.0, 340.0, n_systems)
+    t0 = time.perf_counter()
+    tops = [pm.Topology.generate(int(n), seed=seed + 1 + k) for k, n in enumerate(sizes)]
+    params = [pm.default_params(seed=seed + 1000 + k, stream_id=k, temperature_K=float(T), end_temperature_K=float(T))
+              for k, T in enumerate(temps)]
+    total = int(sizes.sum())
+    log(f"ensemble: {n_systems} systems, {total} atoms generated in {time.perf_counter() - t0:.1f} s")
+    ens = pm.Ensemble(tops, params)
+    ens.minimize(1000, 1.0)
+    starts = [ens.get_positions(k) for k in range(n_systems)]
+    ens.step(2 * graph_steps, graph_steps)  # warm-up, captures the graph
+    # per-graph-launch latency: each replay runs graph_steps steps of every system
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps // graph_steps):
+        t1 = time.perf_counter()
+        ens.step(graph_steps, graph_steps)
+        lat.append((time.perf_counter() - t1) / graph_steps)
+    wall_b = time.perf_counter() - t0
+    done_b = (steps // graph_steps) * graph_steps
+    bad = [k for k in range(n_systems) if ens.stats(k)["status"] != 0]
+    ens.close()
+    batched = total * done_b / wall_b
+    # plain path: the same systems one at a time, same starting positions
+    wall_p = 0.0
+    plain_lat = []
+    for k in range(n_systems):
+        with pm.FusedEngine(tops[k], params[k]) as eng:
+            eng.set_state(x=starts[k])
+            eng.step(2 * graph_steps)
+            t1 = time.perf_counter()
+            eng.step(done_b)
+            dt = time.perf_counter() - t1
+            wall_p += dt
+            plain_lat.append(dt / done_b)
+    plain = total * done_b / wall_p
+    lat = np.array(lat) * 1e3
+    return {"n_systems": n_systems, "atoms": total, "steps": done_b, "graph_steps": graph_steps,
+            "atom_steps_per_s": round(batched, 1), "plain_atom_steps_per_s": round(plain, 1),
+            "speedup_vs_plain": round(batched / plain, 3),
+            "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                "p99": round(float(np.percentile(lat, 99)), 4)},
+            "plain_step_latency_ms_all_systems_sequential": round(float(sum(plain_lat)) * 1e3, 4),
+            "systems_with_faults": bad,
+            "sizes": "log-uniform 500-20000 atoms", "temperatures_K": "uniform 280-340"}
+
+
 def traffic_per_launch():
     path = os.path.join(ROOT, "profiles", "nonbonded_25k_traffic.json")
     if not os.path.exists(path):
@@ -98,6 +157,9 @@ def main():
     ap.add_argument("--atoms", type=int, default=25000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--sample-every", type=int, default=10, help="nonbonded timing sample interval (steps)")
+    ap.add_argument("--ensemble-systems", type=int, default=256, help="batched ensemble size (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=400)
+    ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -156,6 +218,10 @@ def main():
         "cpu_baseline": None,
     }
     eng.close()
+    out["ensemble"] = None
+    if args.ensemble_systems > 0:
+        out["ensemble"] = ensemble_leg(pm, args.ensemble_systems, args.ensemble_steps, args.graph_steps)
+        log(f"ensemble: {out['ensemble']}")
     if args.cpu_seconds > 0:
         try:
             out["cpu_baseline"] = cpu_baseline(pm, top, x_min.astype(np.float64), v0.astype(np.float64), params,

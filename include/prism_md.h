@@ -278,6 +278,36 @@ int prism_md_count_pairs(PrismMdHandle *h, uint64_t *n_pairs);
  * measured with HIP events on that stream. */
 int prism_md_time_nonbonded(PrismMdHandle *h, int iters, double *mean_ms);
 
+/* ---- batched ensemble ------------------------------------------------------
+ * Many independent systems of uneven size in one engine: each phase of the
+ * step is one launch over all systems, and batch_step replays a captured
+ * hipGraph of graph_steps steps.  Every system keeps its own parameters,
+ * Philox key (seed, stream_id), schedule, rebuild decisions and fault record;
+ * its trajectory is bitwise the one it would follow alone.  The reference's
+ * batching precedent is the legacy descriptor + flatten design
+ * (crates/prism-gpu/src/kernels/amber_simd_batch.cu:8-22, :822-900); its
+ * production engine runs one system per engine (replicas disabled,
+ * fused_engine.rs:18307). */
+typedef struct PrismMdBatch PrismMdBatch;
+int prism_md_batch_create(const PrismMdTopologyV1 *const *topologies, const PrismMdParamsV1 *params,
+                          int32_t n_systems, void *hip_stream, PrismMdBatch **out);
+int prism_md_batch_destroy(PrismMdBatch *b);
+int prism_md_batch_size(PrismMdBatch *b, int32_t *n_systems, int64_t *n_atoms);
+/* n_steps steps of every active system; graph_steps > 0 captures that many
+ * steps once into a hipGraph and replays it (the remainder runs eagerly). */
+int prism_md_batch_step(PrismMdBatch *b, uint64_t n_steps, int32_t graph_steps);
+/* Joint FIRE minimisation of all active systems (see prism_md_minimize). */
+int prism_md_batch_minimize(PrismMdBatch *b, int max_iter, double f_tol, double *f_max);
+/* Inactive systems keep their state and are skipped by every kernel. */
+int prism_md_batch_set_active(PrismMdBatch *b, int32_t system, int active);
+int prism_md_batch_set_state(PrismMdBatch *b, int32_t system, const float *x, const float *v, int flags);
+int prism_md_batch_get_state(PrismMdBatch *b, int32_t system, float *x, float *v, int flags);
+int prism_md_batch_compute_forces(PrismMdBatch *b, int32_t system, unsigned term_mask, float *F_out,
+                                  double *E_terms_out);
+int prism_md_batch_energies(PrismMdBatch *b, int32_t system, double out[3]);
+int prism_md_batch_get_stats(PrismMdBatch *b, int32_t system, PrismMdStatsV1 *stats);
+const char *prism_md_batch_last_error(PrismMdBatch *b);
+
 /* Sample the nonbonded kernel inside prism_md_step: with every > 0, HIP
  * events on the handle's stream bracket the nonbonded launch of every
  * every-th step from now on (at most 1024 samples per prism_md_step call).

@@ -462,23 +462,31 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
         S.fault_atom = -1;
     }
     if ((rc = upload(P_.sst, st))) return rc;
-    launch_kinetic(P_, d_red_, stream_);  // initial COM velocity (zero by construction)
+    launch_kinetic(P_, d_red_, -2, stream_);  // initial COM velocity (zero by construction)
     if ((rc = refresh_all())) return rc;
     return check_status();
 }
 
 // Rebuild the lists at the current positions and recompute the carried force.
-int Engine::refresh_all() {
+int Engine::refresh_all() { return refresh(-1); }
+
+// Rebuild the lists of system `only` (-1: every system) at the current
+// positions with a re-sort, re-claim the unit list and recompute the carried
+// force.  Systems not rebuilt keep their rows, so their forces -- and their
+// trajectories -- are bitwise unchanged.
+int Engine::refresh(int only) {
     static const int ones[2] = {1, 1};  // rebuild_flag, resort_flag
     static_assert(offsetof(SysState, resort_flag) == offsetof(SysState, rebuild_flag) + sizeof(int), "flag layout");
     for (int s = 0; s < (int)sys_.size(); ++s) {
+        if (only >= 0 && s != only) continue;
         if (int rc = hipcheck(hipMemcpyAsync((char *)(P_.sst + s) + offsetof(SysState, rebuild_flag), ones, sizeof(ones),
                                              hipMemcpyHostToDevice, stream_),
                               "flag"))
             return rc;
     }
     launch_sort(P_, stream_);
-    launch_rows(P_, -1, stream_);
+    launch_rows(P_, stream_);
+    launch_consume(P_, -1, stream_);
     full_force();
     return hipcheck(hipGetLastError(), "refresh");
 }
@@ -493,7 +501,7 @@ void Engine::full_force() {
 void Engine::launch_step(const float *noise_n, const float *noise_np1) {
     launch_first_half(P_, noise_n, stream_);
     launch_sort(P_, stream_);
-    launch_rows(P_, 0, stream_);
+    launch_rows(P_, stream_);
     const bool sample = timing_every_ > 0 && timing_counter_++ % (uint64_t)timing_every_ == 0 &&
                         2 * ev_used_ + 1 < ev_pool_.size();
     if (sample) hipEventRecord(ev_pool_[2 * ev_used_], stream_);
@@ -562,10 +570,10 @@ int Engine::set_state(int s, const float *x, const float *v, int flags) {
         if (int rc = hipcheck(hipMemcpyAsync(P_.v + H.atom_begin, pk.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream_),
                               "set v"))
             return rc;
-        launch_kinetic(P_, d_red_, stream_);
+        launch_kinetic(P_, d_red_, s, stream_);
     }
     if (x)
-        if (int rc = refresh_all()) return rc;
+        if (int rc = refresh(s)) return rc;
     return check_status();
 }
 
@@ -689,9 +697,15 @@ int Engine::step_graph(uint64_t n_steps, int graph_steps) {
         if (graph_) hipGraphExecDestroy(graph_);
         graph_ = nullptr;
         hipGraph_t g;
+        const int every = timing_every_;
+        const uint64_t evals = force_evals_;
+        timing_every_ = 0;  // no event records inside the captured graph
         if (int rc = hipcheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture")) return rc;
         for (int k = 0; k < graph_steps; ++k) launch_step(nullptr, nullptr);
-        if (int rc = hipcheck(hipStreamEndCapture(stream_, &g), "end capture")) return rc;
+        const hipError_t ce = hipStreamEndCapture(stream_, &g);
+        timing_every_ = every;
+        force_evals_ = evals;
+        if (int rc = hipcheck(ce, "end capture")) return rc;
         const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (int rc = hipcheck(e, "instantiate")) return rc;
@@ -710,7 +724,7 @@ int Engine::step_graph(uint64_t n_steps, int graph_steps) {
 int Engine::energies(int s, double out[3]) {
     double E[PRISM_MD_N_TERMS];
     if (int rc = compute_forces(PRISM_MD_TERM_ALL, s, nullptr, nullptr, E)) return rc;
-    launch_kinetic(P_, d_red_, stream_);
+    launch_kinetic(P_, d_red_, -1, stream_);
     double red[4];
     if (int rc = hipcheck(hipMemcpyAsync(red, d_red_ + 4 * s, sizeof(red), hipMemcpyDeviceToHost, stream_), "ke copy")) return rc;
     if (int rc = hipcheck(hipStreamSynchronize(stream_), "sync")) return rc;
@@ -844,7 +858,8 @@ int Engine::minimize(int max_iter, double f_tol, double *e_final, double *f_max)
         }
         launch_fire_update(P_, d_vmin_, cv, cf, (float)dt, 0.1f, stream_);
         launch_sort(P_, stream_);
-        launch_rows(P_, -1, stream_);
+        launch_rows(P_, stream_);
+        launch_consume(P_, -1, stream_);
     }
     launch_snap_hydrogens(P_, stream_);
     if (int rc = refresh_all()) return rc;
@@ -886,8 +901,24 @@ int Engine::stats(int s, PrismMdStatsV1 *out) {
 
 int Engine::set_active(int s, bool active) {
     if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    if ((desc_[s].active != 0) == active) return PRISM_MD_OK;
     desc_[s].active = active ? 1 : 0;
-    return hipcheck(hipMemcpyAsync(d_desc_ + s, &desc_[s], sizeof(SysDesc), hipMemcpyHostToDevice, stream_), "activate");
+    if (int rc = hipcheck(hipMemcpyAsync(d_desc_ + s, &desc_[s], sizeof(SysDesc), hipMemcpyHostToDevice, stream_),
+                          "activate"))
+        return rc;
+    if (active) {
+        // re-sort and rebuild this system, re-claim the unit list, and give it
+        // its carried force before its first step
+        if (int rc = refresh(s)) return rc;
+    } else {
+        // drop its units: every active block re-claims, no row changes
+        const EngineCounters c{0u, 1, 0, 0};
+        if (int rc = hipcheck(hipMemcpyAsync(P_.counters, &c, sizeof(c), hipMemcpyHostToDevice, stream_), "counters"))
+            return rc;
+        launch_rows(P_, stream_);
+        launch_consume(P_, -1, stream_);
+    }
+    return check_status();
 }
 
 }  // namespace prism
@@ -979,6 +1010,83 @@ int prism_md_minimize(PrismMdHandle *h, int max_iter, double f_tol, double *e_fi
 int prism_md_kernel_timing(PrismMdHandle *h, int every, double *mean_ms, uint64_t *samples) {
     return h ? h->eng.kernel_timing(every, mean_ms, samples) : PRISM_MD_E_INVALID;
 }
+
+// ---- batched ensemble ---------------------------------------------------------
+struct PrismMdBatch {
+    prism::Engine eng;
+};
+
+int prism_md_batch_create(const PrismMdTopologyV1 *const *topologies, const PrismMdParamsV1 *params, int32_t n_systems,
+                          void *hip_stream, PrismMdBatch **out) {
+    if (!topologies || !params || !out || n_systems < 1) return PRISM_MD_E_INVALID;
+    *out = nullptr;
+    std::vector<prism::SystemSpec> specs;
+    specs.reserve((size_t)n_systems);
+    for (int32_t s = 0; s < n_systems; ++s) {
+        if (!topologies[s]) return PRISM_MD_E_INVALID;
+        specs.push_back(prism::SystemSpec{topologies[s], params[s]});
+    }
+    PrismMdBatch *b = new (std::nothrow) PrismMdBatch();
+    if (!b) return PRISM_MD_E_NOMEM;
+    const int rc = b->eng.init(specs, hip_stream);
+    if (rc != PRISM_MD_OK) {
+        g_create_error = b->eng.last_error();
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return PRISM_MD_OK;
+}
+
+int prism_md_batch_destroy(PrismMdBatch *b) {
+    delete b;
+    return PRISM_MD_OK;
+}
+
+int prism_md_batch_size(PrismMdBatch *b, int32_t *n_systems, int64_t *n_atoms) {
+    if (!b) return PRISM_MD_E_INVALID;
+    int64_t total = 0;
+    for (int s = 0; s < b->eng.n_systems(); ++s) total += b->eng.n_atoms(s);
+    if (n_systems) *n_systems = b->eng.n_systems();
+    if (n_atoms) *n_atoms = total;
+    return PRISM_MD_OK;
+}
+
+int prism_md_batch_step(PrismMdBatch *b, uint64_t n_steps, int32_t graph_steps) {
+    if (!b) return PRISM_MD_E_INVALID;
+    return graph_steps > 0 ? b->eng.step_graph(n_steps, graph_steps) : b->eng.step(n_steps, nullptr);
+}
+
+int prism_md_batch_minimize(PrismMdBatch *b, int max_iter, double f_tol, double *f_max) {
+    return b ? b->eng.minimize(max_iter, f_tol, nullptr, f_max) : PRISM_MD_E_INVALID;
+}
+
+int prism_md_batch_set_active(PrismMdBatch *b, int32_t system, int active) {
+    return b ? b->eng.set_active(system, active != 0) : PRISM_MD_E_INVALID;
+}
+
+int prism_md_batch_set_state(PrismMdBatch *b, int32_t system, const float *x, const float *v, int flags) {
+    return b ? b->eng.set_state(system, x, v, flags) : PRISM_MD_E_INVALID;
+}
+
+int prism_md_batch_get_state(PrismMdBatch *b, int32_t system, float *x, float *v, int flags) {
+    return b ? b->eng.get_state(system, x, v, flags) : PRISM_MD_E_INVALID;
+}
+
+int prism_md_batch_compute_forces(PrismMdBatch *b, int32_t system, unsigned term_mask, float *F_out,
+                                  double *E_terms_out) {
+    return b ? b->eng.compute_forces(term_mask, system, F_out, nullptr, E_terms_out) : PRISM_MD_E_INVALID;
+}
+
+int prism_md_batch_energies(PrismMdBatch *b, int32_t system, double out[3]) {
+    return b && out ? b->eng.energies(system, out) : PRISM_MD_E_INVALID;
+}
+
+int prism_md_batch_get_stats(PrismMdBatch *b, int32_t system, PrismMdStatsV1 *stats) {
+    return b ? b->eng.stats(system, stats) : PRISM_MD_E_INVALID;
+}
+
+const char *prism_md_batch_last_error(PrismMdBatch *b) { return b ? b->eng.last_error() : g_create_error.c_str(); }
 
 int prism_md_time_nonbonded(PrismMdHandle *h, int iters, double *mean_ms) {
     return h && mean_ms && iters > 0 ? h->eng.time_nonbonded(iters, mean_ms) : PRISM_MD_E_INVALID;

@@ -63,6 +63,7 @@ class Engine {
     int hipcheck(hipError_t e, const char *what);
     int d2h(void *dst, const void *src, size_t bytes);  // stream-ordered copy + synchronise
     int refresh_all();            // rebuild lists + forces at current positions
+    int refresh(int only);        // the same for one system (-1: all)
     void launch_step(const float *noise_n, const float *noise_np1);
     void full_force();
     void harvest_timing();

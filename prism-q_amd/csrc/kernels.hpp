@@ -67,10 +67,12 @@ struct RefGrid {                    // reference cell grid of the pair-list pari
 };
 
 // Rebuild of the systems whose rebuild flag is set: re-sort or box refresh,
-// then rows and the unit list.  step_offset: 0 inside a step (lists built at
-// x_(s+1) after SHAKE of step s), -1 for a rebuild between steps (at x_s).
+// then rows and the unit list.  Inside a step the flags are consumed by
+// k_second_half (lists built at x_(s+1) after SHAKE of step s); after a
+// rebuild between steps launch_consume(step_offset = -1) does it (at x_s).
 void launch_sort(const DevPtrs &P, hipStream_t s);
-void launch_rows(const DevPtrs &P, int step_offset, hipStream_t s);
+void launch_rows(const DevPtrs &P, hipStream_t s);
+void launch_consume(const DevPtrs &P, int step_offset, hipStream_t s);
 void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s);
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);
@@ -81,7 +83,9 @@ void launch_zero_forces(const DevPtrs &P, hipStream_t s);
 void launch_gather(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 // Per-system FP64 energy sums of the last evaluation into SysState.energy.
 void launch_energy_reduce(const DevPtrs &P, hipStream_t s);
-void launch_kinetic(const DevPtrs &P, double *ke_out, hipStream_t s);
+// ke_out[S][4] = (KE, COM velocity); com_sys: system whose carried COM
+// velocity is also set (-2 all, -1 none).
+void launch_kinetic(const DevPtrs &P, double *ke_out, int com_sys, hipStream_t s);
 void launch_emit_pairs(const DevPtrs &P, int mode, RefGrid g, int *out_i, int *out_j, unsigned long long *count,
                        unsigned long long cap, hipStream_t s);
 void launch_count_pairs(const DevPtrs &P, unsigned long long *count, hipStream_t s);

@@ -75,7 +75,8 @@ __device__ inline void refresh_block(const DevPtrs &P, int b, int lane) {
 // Workgroups [0, n_sys): one per system, a full re-sort when the system's
 // rebuild is due and resort_due() holds.  Workgroups from n_sys on: one wave
 // per block, a box refresh of blocks whose system rebuilds without re-sorting.
-// Nothing here writes the flags it reads; the row builder consumes them.
+// Nothing here writes the flags it reads; they are consumed at the end of the
+// step (k_second_half) or by k_consume after a rebuild between steps.
 __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
     __shared__ float red[6][SORT_WAVES];
     __shared__ unsigned int hist[256];
@@ -91,7 +92,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
         const SysState &S = P.sst[s];
         if (!D.active || !S.rebuild_flag || resort_due(D, S)) return;
         refresh_block(P, b, lane);
-        if (lane == 0 && b == D.block_begin) atomicExch(&P.counters->any_dirty, 1);
+        if (lane == 0 && b == D.block_begin) {
+            atomicExch(&P.counters->any_dirty, 1);
+            atomicExch(&P.counters->n_units, 0);  // every active block re-claims its units
+        }
         return;
     }
     const int s = blockIdx.x;
@@ -222,15 +226,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
 
     // ---- block boxes over real atoms
     for (int b = wave; b < D.n_blocks; b += SORT_WAVES) refresh_block(P, D.block_begin + b, lane);
-    if (tid == 0) atomicExch(&P.counters->any_dirty, 1);
+    if (tid == 0) {
+        atomicExch(&P.counters->any_dirty, 1);
+        atomicExch(&P.counters->n_units, 0);  // every active block re-claims its units
+    }
 }
 
 // One wave per i-block: the row of j-blocks whose build-time boxes lie within
 // the list cutoff (full list, both directions, ascending j) and the mask slot
 // of each tile whose j-block holds a skipped pair (or is the block itself).
 // The masks themselves are rebuilt only when the atoms were re-sorted.
-__device__ void build_row(const DevPtrs &P, int b, bool resort, int wave, int lane, int (*row_lds)[MAX_ROW],
-                          int (*flag_lds)[MAX_ROW], unsigned long long (*mask_lds)[MAX_MASKS][64]) {
+__device__ int build_row(const DevPtrs &P, int b, bool resort, int wave, int lane, int (*row_lds)[MAX_ROW],
+                         int (*flag_lds)[MAX_ROW], unsigned long long (*mask_lds)[MAX_MASKS][64]) {
     const int s = P.block_sys[b];
     const SysDesc &D = P.sys[s];
     SysState &S = P.sst[s];
@@ -334,71 +341,42 @@ __device__ void build_row(const DevPtrs &P, int b, bool resort, int wave, int la
         }
         P.row_m[(size_t)b * MAX_ROW + t] = m;
     }
+    return count;
 }
 
 constexpr int ROW_WAVES = 2;
-__global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P, int step_offset) {
+// One wave per block.  On a step where some system rebuilds: blocks of a
+// rebuilding system rebuild their row; every active block then claims a range
+// of the unit list with one atomic add and writes its units there.  The claim
+// order varies, but no result depends on it: the gather reads each block's
+// units through unit_off in chunk order.
+__global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = blockIdx.x * ROW_WAVES + wave;
     __shared__ int row_lds[ROW_WAVES][MAX_ROW];
     __shared__ int flag_lds[ROW_WAVES][MAX_ROW];
     __shared__ unsigned long long mask_lds[ROW_WAVES][MAX_MASKS][64];
-    __shared__ int last;
-    // the sort of this step raised any_dirty when it re-sorted some system;
-    // without it no row changes and the unit list stays valid
-    if (P.counters->any_dirty == 0) return;
-    if (b < P.n_blocks_arena) {
-        const int s = P.block_sys[b];
-        if (s >= 0 && P.sys[s].active && P.sst[s].rebuild_flag)
-            build_row(P, b, resort_due(P.sys[s], P.sst[s]), wave, lane, row_lds, flag_lds, mask_lds);
+    if (P.counters->any_dirty == 0 || b >= P.n_blocks_arena) return;
+    const int s = P.block_sys[b];
+    if (s < 0 || !P.sys[s].active) return;
+    int cnt;
+    if (P.sst[s].rebuild_flag) {
+        const int tiles = build_row(P, b, resort_due(P.sys[s], P.sst[s]), wave, lane, row_lds, flag_lds, mask_lds);
+        cnt = QUARTERS * ((tiles + CHUNK - 1) / CHUNK);
+    } else {
+        cnt = P.unit_count[b];
     }
-    // Last-block handshake (no waiting): when any row changed, the workgroup
-    // that retires last rebuilds the compact list of nonbonded work units,
-    // block after block in arena order.
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&P.counters->rows_done, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    __shared__ int base;
-    if (threadIdx.x == 0) base = 0;
-    __syncthreads();
-    {
-        for (int b0 = 0; b0 < P.n_blocks_arena; b0 += 64 * ROW_WAVES) {
-            const int bb = b0 + threadIdx.x;
-            int cnt = 0;
-            if (bb < P.n_blocks_arena) {
-                const int s = P.block_sys[bb];
-                cnt = (s >= 0 && P.sys[s].active) ? P.unit_count[bb] : 0;
-            }
-            // workgroup exclusive scan of cnt
-            __shared__ int scan[64 * ROW_WAVES];
-            scan[threadIdx.x] = cnt;
-            __syncthreads();
-            for (int o = 1; o < 64 * ROW_WAVES; o <<= 1) {
-                const int v = threadIdx.x >= o ? scan[threadIdx.x - o] : 0;
-                __syncthreads();
-                scan[threadIdx.x] += v;
-                __syncthreads();
-            }
-            const int off = base + scan[threadIdx.x] - cnt;
-            if (bb < P.n_blocks_arena) P.unit_off[bb] = off;
-            const int per_q = cnt / QUARTERS;  // chunks per quarter
-            for (int k = 0; k < cnt; ++k) {
-                const int q = k / per_q, c = k % per_q;
-                P.units[off + k] = (bb << 8) | (q << 6) | c;
-            }
-            __syncthreads();
-            if (threadIdx.x == 64 * ROW_WAVES - 1) base = off + cnt;
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            P.counters->n_units = base;
-            P.counters->any_dirty = 0;
-        }
-    }
-    // every other workgroup has finished reading the flags: consume them
+    int off = 0;
+    if (lane == 0) off = atomicAdd(&P.counters->n_units, cnt);
+    off = __shfl(off, 0);
+    if (lane == 0) P.unit_off[b] = off;
+    const int per_q = cnt / QUARTERS;  // chunks per quarter
+    for (int k = lane; k < cnt; k += 64) P.units[off + k] = (b << 8) | ((k / per_q) << 6) | (k % per_q);
+}
+
+// Consume the rebuild flags after a rebuild between steps (the integrator
+// consumes them at the end of a step).
+__global__ __launch_bounds__(256) void k_consume(DevPtrs P, int step_offset) {
     for (int s = threadIdx.x; s < P.n_sys; s += blockDim.x) {
         SysState &S = P.sst[s];
         if (!P.sys[s].active || !S.rebuild_flag) continue;
@@ -407,7 +385,7 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P, int step_off
         S.rebuild_step = (long long)S.step + step_offset;
         S.n_rebuilds += 1;
     }
-    if (threadIdx.x == 0) P.counters->rows_done = 0;
+    if (threadIdx.x == 0) P.counters->any_dirty = 0;
 }
 
 }  // namespace
@@ -417,9 +395,13 @@ void launch_sort(const DevPtrs &P, hipStream_t st) {
     hipLaunchKernelGGL(k_sort, dim3(P.n_sys + refresh), dim3(SORT_THREADS), 0, st, P);
 }
 
-void launch_rows(const DevPtrs &P, int step_offset, hipStream_t st) {
+void launch_rows(const DevPtrs &P, hipStream_t st) {
     const int grid = (P.n_blocks_arena + ROW_WAVES - 1) / ROW_WAVES;
-    hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), 0, st, P, step_offset);
+    hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), 0, st, P);
+}
+
+void launch_consume(const DevPtrs &P, int step_offset, hipStream_t st) {
+    hipLaunchKernelGGL(k_consume, dim3(1), dim3(256), 0, st, P, step_offset);
 }
 
 }  // namespace prism

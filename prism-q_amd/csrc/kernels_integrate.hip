@@ -318,14 +318,26 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
                 for (int c = 0; c < 3; ++c) acc[c] += P.com_partial[(size_t)w * 3 + c];
             for (int c = 0; c < 3; ++c) S.com_v[c] = acc[c] * D.inv_total_mass;
         }
+        if (S.rebuild_flag) {  // this step's rebuild is done: consume its flags
+            S.rebuild_flag = 0;
+            S.resort_flag = 0;
+            S.rebuild_step = (long long)step;
+            S.n_rebuilds += 1;
+        }
         S.step = step + 1;
     }
-    if (threadIdx.x == 0) P.counters->done_blocks = 0;
+    if (threadIdx.x == 0) {
+        P.counters->done_blocks = 0;
+        P.counters->any_dirty = 0;
+    }
 }
 
 // Per-system reductions in FP64 (one workgroup per system, fixed order):
 // kinetic energy and centre-of-mass velocity.
-__global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs P, double *out /*[S][4]*/) {
+// Per-system kinetic energy and COM velocity.  The COM velocity carried to the
+// next COM-removal step is overwritten only for com_sys (a system index, -2 for
+// all, -1 for none), so reading energies never perturbs a trajectory.
+__global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs P, double *out /*[S][4]*/, int com_sys) {
     const int s = blockIdx.x;
     if (s >= P.n_sys) return;
     const SysDesc &D = P.sys[s];
@@ -350,7 +362,7 @@ __global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs P, double *out /*[S]
         out[s * 4 + 0] = r[0] / KCAL_TO_AMU_A2_PS2;
         for (int c = 0; c < 3; ++c) {
             out[s * 4 + 1 + c] = r[1 + c] * D.inv_total_mass;
-            P.sst[s].com_v[c] = r[1 + c] * D.inv_total_mass;
+            if (com_sys == -2 || com_sys == s) P.sst[s].com_v[c] = r[1 + c] * D.inv_total_mass;
         }
     }
 }
@@ -367,8 +379,8 @@ void launch_second_half(const DevPtrs &P, const float *noise, hipStream_t st) {
     hipLaunchKernelGGL(k_second_half, dim3(grid), dim3(64 * IW), 0, st, P, noise);
 }
 
-void launch_kinetic(const DevPtrs &P, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_sys_reduce, dim3(P.n_sys), dim3(256), 0, st, P, out);
+void launch_kinetic(const DevPtrs &P, double *out, int com_sys, hipStream_t st) {
+    hipLaunchKernelGGL(k_sys_reduce, dim3(P.n_sys), dim3(256), 0, st, P, out, com_sys);
 }
 
 }  // namespace prism

@@ -76,7 +76,7 @@ struct SysDesc {
 struct SysState {
     unsigned long long step;        // index of the step being executed
     long long rebuild_step;         // step whose positions the pair list was built at (-1: before step 0)
-    int rebuild_flag;               // set by the integrator or the host, consumed by the row builder
+    int rebuild_flag;               // set by the integrator or the host, consumed at the end of the step
     int resort_flag;                // set by the host: the next rebuild re-sorts (positions replaced)
     int status;                     // ST_* bits
     int n_rebuilds;
@@ -110,9 +110,9 @@ __host__ __device__ inline bool resort_due(const SysDesc &D, const SysState &S) 
 // Global engine counters (device-written).
 struct EngineCounters {
     unsigned int done_blocks;       // last-block handshake of the integrator
-    unsigned int rows_done;         // last-block handshake of the row builder
-    int any_dirty;                  // some system rebuilt its rows this step
-    int n_units;                    // nonbonded work units in the unit list
+    int any_dirty;                  // some system rebuilds its rows this step
+    int n_units;                    // nonbonded work units claimed by the row builder
+    int pad;
 };
 
 struct Group {                      // constraint group

@@ -132,6 +132,19 @@ def md_lib():
             "prism_md_count_pairs": ([H, C.POINTER(C.c_uint64)], C.c_int),
             "prism_md_time_nonbonded": ([H, C.c_int, C.POINTER(C.c_double)], C.c_int),
             "prism_md_kernel_timing": ([H, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)], C.c_int),
+            "prism_md_batch_create": ([C.POINTER(C.POINTER(TopologyV1)), C.POINTER(ParamsV1), C.c_int32, C.c_void_p,
+                                       C.POINTER(H)], C.c_int),
+            "prism_md_batch_destroy": ([H], C.c_int),
+            "prism_md_batch_size": ([H, C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
+            "prism_md_batch_step": ([H, C.c_uint64, C.c_int32], C.c_int),
+            "prism_md_batch_minimize": ([H, C.c_int, C.c_double, C.POINTER(C.c_double)], C.c_int),
+            "prism_md_batch_set_active": ([H, C.c_int32, C.c_int], C.c_int),
+            "prism_md_batch_set_state": ([H, C.c_int32, _f32p, _f32p, C.c_int], C.c_int),
+            "prism_md_batch_get_state": ([H, C.c_int32, _f32p, _f32p, C.c_int], C.c_int),
+            "prism_md_batch_compute_forces": ([H, C.c_int32, C.c_uint, _f32p, C.POINTER(C.c_double)], C.c_int),
+            "prism_md_batch_energies": ([H, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+            "prism_md_batch_get_stats": ([H, C.c_int32, C.POINTER(StatsV1)], C.c_int),
+            "prism_md_batch_last_error": ([H], C.c_char_p),
             "prism_md_minimize": ([H, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         }
         for name, (args, res) in sig.items():
@@ -388,3 +401,100 @@ class FusedEngine:
         ms = C.c_double()
         self._check(self.lib.prism_md_time_nonbonded(self.h, iters, C.byref(ms)), "time_nonbonded")
         return float(ms.value)
+
+
+class Ensemble:
+    """Many independent systems in one engine (prism_md_batch_*): one launch
+    per phase over all systems, stepped through a captured hipGraph.  Each
+    system keeps its own parameters, noise stream, rebuilds and faults; its
+    trajectory is bitwise the one it follows alone."""
+
+    def __init__(self, topologies, params=None, hip_stream: int | None = None):
+        self.lib = md_lib()
+        self.topologies = list(topologies)
+        n = len(self.topologies)
+        if params is None:
+            params = [default_params() for _ in range(n)]
+        elif isinstance(params, ParamsV1):
+            params = [params] * n
+        self.params = list(params)
+        self._structs = [t.struct() for t in self.topologies]
+        tops = (C.POINTER(TopologyV1) * n)(*[C.pointer(s) for s in self._structs])
+        prm = (ParamsV1 * n)(*self.params)
+        h = C.c_void_p()
+        rc = self.lib.prism_md_batch_create(tops, prm, n, C.c_void_p(hip_stream) if hip_stream else None, C.byref(h))
+        if rc != OK:
+            self.h = None
+            raise PrismMdError(rc, "prism_md_batch_create: " + self.lib.prism_md_batch_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != OK:
+            raise PrismMdError(rc, f"{what}: {self.lib.prism_md_batch_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.prism_md_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def n_systems(self) -> int:
+        return len(self.topologies)
+
+    def n_atoms(self) -> int:
+        n, a = C.c_int32(), C.c_int64()
+        self._check(self.lib.prism_md_batch_size(self.h, C.byref(n), C.byref(a)), "batch_size")
+        return int(a.value)
+
+    def step(self, n_steps: int, graph_steps: int = 10):
+        self._check(self.lib.prism_md_batch_step(self.h, n_steps, graph_steps), "batch_step")
+
+    def minimize(self, max_iter: int = 2000, f_tol: float = 1.0) -> float:
+        f = C.c_double()
+        self._check(self.lib.prism_md_batch_minimize(self.h, max_iter, f_tol, C.byref(f)), "batch_minimize")
+        return f.value
+
+    def set_active(self, s: int, active: bool):
+        self._check(self.lib.prism_md_batch_set_active(self.h, s, int(bool(active))), "batch_set_active")
+
+    def set_state(self, s: int, x=None, v=None):
+        xa = None if x is None else np.ascontiguousarray(x, np.float32).reshape(-1)
+        va = None if v is None else np.ascontiguousarray(v, np.float32).reshape(-1)
+        self._check(self.lib.prism_md_batch_set_state(self.h, s, _ptr(xa, C.c_float), _ptr(va, C.c_float), 0),
+                    "batch_set_state")
+
+    def get_positions(self, s: int) -> np.ndarray:
+        x = np.zeros(3 * self.topologies[s].n_atoms, np.float32)
+        self._check(self.lib.prism_md_batch_get_state(self.h, s, _ptr(x, C.c_float), None, 0), "batch_get_state")
+        return x.reshape(-1, 3)
+
+    def get_velocities(self, s: int) -> np.ndarray:
+        v = np.zeros(3 * self.topologies[s].n_atoms, np.float32)
+        self._check(self.lib.prism_md_batch_get_state(self.h, s, None, _ptr(v, C.c_float), 0), "batch_get_state")
+        return v.reshape(-1, 3)
+
+    def compute_forces(self, s: int, mask: int = TERM_ALL):
+        F = np.zeros(3 * self.topologies[s].n_atoms, np.float32)
+        E = (C.c_double * 5)()
+        self._check(self.lib.prism_md_batch_compute_forces(self.h, s, mask, _ptr(F, C.c_float), E),
+                    "batch_compute_forces")
+        return F.reshape(-1, 3), np.array(list(E))
+
+    def energies(self, s: int):
+        out = (C.c_double * 3)()
+        self._check(self.lib.prism_md_batch_energies(self.h, s, out), "batch_energies")
+        return {"potential": out[0], "kinetic": out[1], "temperature": out[2]}
+
+    def stats(self, s: int) -> dict:
+        st = StatsV1()
+        self._check(self.lib.prism_md_batch_get_stats(self.h, s, C.byref(st)), "batch_stats")
+        return {f: getattr(st, f) for f, _ in StatsV1._fields_ if f != "reserved"}
