@@ -22,11 +22,15 @@ One JSON line on stdout:
   ensemble     256 systems of log-uniform size in [500, 20000] atoms at
                280-340 K in one batched engine stepped through a captured
                hipGraph: aggregate atom-steps/s, p50/p99 per-step latency,
-               and the same systems run one at a time on the plain path.
+               and the same systems run one at a time on the plain path;
+  serving      continuous batching of Poisson-arriving jobs admitted into a
+               slot pool sized to an HBM budget: p50/p99 per-step latency,
+               aggregate atom-steps/s, jobs done, admission wait.
 """
 from __future__ import annotations
 
 import argparse
+import collections
 import json
 import os
 import sys
@@ -141,6 +145,94 @@ def ensemble_leg(pm, n_systems, steps, graph_steps, seed=0x50524953 + 3):
             "sizes": "log-uniform 500-20000 atoms", "temperatures_K": "uniform 280-340"}
 
 
+def serving_leg(pm, seconds, rate, budget_gib, graph_steps, seed=0x50524953 + 5):
+    """Continuous batching: Poisson arrivals of generator systems (sizes
+    log-uniform 300-50,000 atoms, lengths uniform 1,000-100,000 steps at 4 fs,
+    280-340 K, own seeds) admitted into a slot pool sized to an HBM budget;
+    all running jobs advance together, one captured-graph replay per tick."""
+    import queue as queue_mod
+    import threading
+
+    import numpy as np
+    from prism_md.serving import Job, Scheduler, plan_slots
+    rng = np.random.default_rng(seed)
+    template = pm.Topology.generate(4000, seed=seed)
+    classes = [1024, 4096, 16384, 51200]
+    caps_for = {c: pm.slot_caps_for(template, atoms=c, scale=1.3) for c in classes}
+    probe = pm.ServingPool([caps_for[4096]])
+    bytes_per_atom = probe.device_bytes() / 4096.0
+    probe.close()
+    # expected share of log-uniform(300, 50000) jobs per smallest fitting class
+    edges = [300] + [c - 1 for c in classes]
+    shares = [np.log(edges[k + 1] / edges[k]) / np.log(edges[-1] / 300.0) for k in range(len(classes))]
+    counts = plan_slots(budget_gib * 2 ** 30, bytes_per_atom, classes, shares)
+    caps = [caps_for[c] for c, n in zip(classes, counts) for _ in range(n)]
+    pool = pm.ServingPool(caps)
+    sch = Scheduler(pool, graph_steps=graph_steps)
+    # producer: topologies generated off the stepping thread (the generator
+    # releases the interpreter lock), handed over at their arrival times
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 2) + 8))
+    sizes = np.exp(rng.uniform(np.log(300), np.log(50000), len(arrivals))).astype(int)
+    lengths = rng.integers(1000, 100001, len(arrivals))
+    temps = rng.uniform(280.0, 340.0, len(arrivals))
+    ready: queue_mod.Queue = queue_mod.Queue(maxsize=64)
+    stop = threading.Event()
+
+    def produce():
+        for k in range(len(arrivals)):
+            if stop.is_set():
+                return
+            top = pm.Topology.generate(int(sizes[k]), seed=seed + 10 + k)
+            p = pm.default_params(seed=seed + 100000 + k, stream_id=k, temperature_K=float(temps[k]),
+                                  end_temperature_K=float(temps[k]))
+            while not stop.is_set():
+                try:
+                    ready.put((k, Job(k, top, p, int(lengths[k]))), timeout=0.1)
+                    break
+                except queue_mod.Full:
+                    pass
+
+    th = threading.Thread(target=produce, daemon=True)
+    th.start()
+    pending = collections.deque()
+    t0 = time.perf_counter()
+    submitted = 0
+    while True:
+        now = time.perf_counter() - t0
+        if now >= seconds:
+            break
+        while True:
+            try:
+                pending.append(ready.get_nowait())
+            except queue_mod.Empty:
+                break
+        while pending and arrivals[pending[0][0]] <= now:
+            sch.submit(pending.popleft()[1])
+            submitted += 1
+        if not sch.tick():
+            time.sleep(0.0005)
+    wall = time.perf_counter() - t0
+    stop.set()
+    rep = sch.report(wall)
+    rep.update({"seconds": round(wall, 1), "arrival_rate_per_s": rate, "jobs_submitted": submitted,
+                "hbm_budget_gib": budget_gib, "pool_bytes": pool.device_bytes(),
+                "slots": {str(c): n for c, n in zip(classes, counts)},
+                "sizes": "log-uniform 300-50000 atoms", "lengths": "uniform 1000-100000 steps",
+                "temperatures_K": "uniform 280-340"})
+    # per-step latency of one median-size job run alone on the plain path
+    done_sizes = sorted(j.n_atoms for j in sch.done + list(sch.running.values()))
+    if done_sizes:
+        n_med = done_sizes[len(done_sizes) // 2]
+        with pm.FusedEngine(pm.Topology.generate(n_med, seed=seed + 7), pm.default_params()) as eng:
+            eng.step(20)
+            t1 = time.perf_counter()
+            eng.step(500)
+            rep["plain_step_latency_ms_median_job"] = round((time.perf_counter() - t1) * 2.0, 4)
+            rep["plain_median_job_atoms"] = n_med
+    pool.close()
+    return rep
+
+
 def traffic_per_launch():
     path = os.path.join(ROOT, "profiles", "nonbonded_25k_traffic.json")
     if not os.path.exists(path):
@@ -160,6 +252,9 @@ def main():
     ap.add_argument("--ensemble-systems", type=int, default=256, help="batched ensemble size (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=400)
     ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--serving-seconds", type=float, default=60.0, help="serving loop run time (0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=1.0, help="job arrivals per second")
+    ap.add_argument("--serving-budget-gib", type=float, default=16.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -222,6 +317,11 @@ def main():
     if args.ensemble_systems > 0:
         out["ensemble"] = ensemble_leg(pm, args.ensemble_systems, args.ensemble_steps, args.graph_steps)
         log(f"ensemble: {out['ensemble']}")
+    out["serving"] = None
+    if args.serving_seconds > 0:
+        out["serving"] = serving_leg(pm, args.serving_seconds, args.serving_rate, args.serving_budget_gib,
+                                     args.graph_steps)
+        log(f"serving: {out['serving']}")
     if args.cpu_seconds > 0:
         try:
             out["cpu_baseline"] = cpu_baseline(pm, top, x_min.astype(np.float64), v0.astype(np.float64), params,

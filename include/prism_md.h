@@ -175,6 +175,16 @@ typedef struct PrismMdStatsV1 {
 
 typedef struct PrismMdHandle PrismMdHandle;
 
+/* Capacity of one slot of a batch or serving pool.  A system fits when
+ * n_atoms < atoms and every other count is <= its capacity (exclusions: the
+ * total length of the exclusion lists, both directions). */
+typedef struct PrismMdSlotCapsV1 {
+    int32_t atoms;
+    int32_t bonds, angles, dihedrals, exceptions;
+    int32_t exclusions;
+    int32_t reserved[2];
+} PrismMdSlotCapsV1;
+
 /* Fills *params with the defaults: dt 0.004 ps (HMR), 300 K, gamma_base
  * 10/ps (fused_engine.rs:13314) with the equilibration friction ramp from
  * 1000/ps over the first 10000 steps (fused_engine.rs:15370-15378), cutoff
@@ -307,6 +317,23 @@ int prism_md_batch_compute_forces(PrismMdBatch *b, int32_t system, unsigned term
 int prism_md_batch_energies(PrismMdBatch *b, int32_t system, double out[3]);
 int prism_md_batch_get_stats(PrismMdBatch *b, int32_t system, PrismMdStatsV1 *stats);
 const char *prism_md_batch_last_error(PrismMdBatch *b);
+
+/* ---- serving pool ----------------------------------------------------------
+ * A batch whose slots start empty: each slot reserves device memory for the
+ * given capacities, so a pool is sized to an HBM budget up front.  Systems
+ * enter a free slot with batch_admit (topology lowered into the slot's ranges,
+ * Maxwell-Boltzmann velocities, own lists and carried force; no allocation,
+ * the captured graph stays valid) and leave with batch_retire; coordinates,
+ * velocities, lists and noise counters stay on the device in between.  A
+ * system's trajectory does not depend on its slot or on the other systems.
+ * Analogue of the reference's stream-ordered pool and batch scheduler
+ * (crates/prism-nhs/src/vram_pool.rs:1-40, crates/prism-nhs/src/batch_scheduler.rs:1-16). */
+int prism_md_slot_caps_for(const PrismMdTopologyV1 *t, PrismMdSlotCapsV1 *caps);
+int prism_md_pool_create(const PrismMdSlotCapsV1 *caps, int32_t n_slots, void *hip_stream, PrismMdBatch **out);
+int prism_md_batch_admit(PrismMdBatch *b, int32_t slot, const PrismMdTopologyV1 *t, const PrismMdParamsV1 *p);
+int prism_md_batch_retire(PrismMdBatch *b, int32_t slot);
+int prism_md_batch_slot_info(PrismMdBatch *b, int32_t slot, PrismMdSlotCapsV1 *caps, int32_t *occupied);
+int prism_md_batch_device_bytes(PrismMdBatch *b, uint64_t *bytes);
 
 /* Sample the nonbonded kernel inside prism_md_step: with every > 0, HIP
  * events on the handle's stream bracket the nonbonded launch of every

@@ -66,6 +66,7 @@ template <class T> T *Engine::dalloc(size_t n) {
         return nullptr;
     }
     allocs_.push_back(p);
+    bytes_ += n * sizeof(T);
     hipMemsetAsync(p, 0, n * sizeof(T), stream_);
     return (T *)p;
 }
@@ -146,11 +147,35 @@ static int validate(const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p, std::s
     return PRISM_MD_OK;
 }
 
-int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
-    for (const auto &sp : systems) {
-        std::string why;
-        if (validate(sp.topo, sp.params, why) != PRISM_MD_OK) return fail(PRISM_MD_E_INVALID, "invalid topology/params: " + why);
-    }
+// ---- arena layout ----------------------------------------------------------
+// Every system lives in a slot: fixed index ranges of every arena array sized
+// by the slot's capacities.  A slot holds at least one padding atom after its
+// last atom, so the CSR end offset of an atom never reaches the next slot.
+// The host keeps a mirror of the arena arrays so a slot can be rewritten and
+// uploaded alone (pool admission) without touching the other slots.
+
+static PrismMdSlotCapsV1 exact_caps(const PrismMdTopologyV1 *t) {
+    PrismMdSlotCapsV1 c{};
+    c.atoms = t->n_atoms + 1;
+    c.bonds = t->n_bonds;
+    c.angles = t->n_angles;
+    c.dihedrals = t->n_dihedrals;
+    c.exceptions = t->n_exceptions;
+    c.exclusions = t->exclusion_offsets[t->n_atoms];
+    return c;
+}
+
+static std::string fits(const PrismMdSlotCapsV1 &c, const PrismMdTopologyV1 *t) {
+    if (t->n_atoms >= c.atoms) return "atoms";
+    if (t->n_bonds > c.bonds) return "bonds";
+    if (t->n_angles > c.angles) return "angles";
+    if (t->n_dihedrals > c.dihedrals) return "dihedrals";
+    if (t->n_exceptions > c.exceptions) return "exceptions";
+    if (t->exclusion_offsets[t->n_atoms] > c.exclusions) return "exclusions";
+    return "";
+}
+
+int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(PRISM_MD_E_NODEVICE, "no HIP device");
     if (stream) {
@@ -159,227 +184,82 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
         if (int rc = hipcheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream")) return rc;
         own_stream_ = true;
     }
-    // ---- arena layout
     int A = 0, W = 0;
-    std::vector<float4> hx, hv;
-    std::vector<float> hmass, hq, hsig, heps;
-    std::vector<int> hatom_sys, hblock_sys, hwave_sys;
-    std::vector<int> skip_off{0}, skip_idx, ex_off{0}, ex_idx;
-    std::vector<Group> hgroups;
-    std::vector<GroupParam> hgp;
-    std::vector<Bond> hb;
-    std::vector<Angle> ha;
-    std::vector<Dihedral> hd;
-    std::vector<Exception14> hx14;
-    for (int s = 0; s < (int)systems.size(); ++s) {
-        const PrismMdTopologyV1 *t = systems[s].topo;
-        const PrismMdParamsV1 &p = systems[s].params;
-        const int n = t->n_atoms;
-        HostSys H{};
-        H.atom_begin = A;
-        H.n_atoms = n;
-        H.n_slots = pad64(n);
-        H.block_begin = A / BLOCK;
-        H.n_blocks = H.n_slots / BLOCK;
-        A += H.n_slots;
-        std::vector<float> v0;
-        maxwell_boltzmann(t, p, v0);
-        double mt = 0.0;
-        for (int i = 0; i < H.n_slots; ++i) {
-            if (i < n) {
-                hx.push_back(make_float4(t->positions[3 * i], t->positions[3 * i + 1], t->positions[3 * i + 2], 0.0f));
-                hv.push_back(make_float4(v0[3 * i], v0[3 * i + 1], v0[3 * i + 2], 0.0f));
-                hmass.push_back(t->masses[i]);
-                hq.push_back(t->charges[i]);
-                hsig.push_back(t->sigma[i]);
-                heps.push_back(t->epsilon[i]);
-                hatom_sys.push_back(s);
-                mt += t->masses[i];
-            } else {
-                hx.push_back(make_float4(0, 0, 0, 0));
-                hv.push_back(make_float4(0, 0, 0, 0));
-                hmass.push_back(1.0f);
-                hq.push_back(0.0f);
-                hsig.push_back(0.0f);
-                heps.push_back(0.0f);
-                hatom_sys.push_back(-1);
-            }
-        }
-        for (int b = 0; b < H.n_blocks; ++b) hblock_sys.push_back(s);
-        // exception partners per atom
-        std::vector<std::vector<int>> expart(n);
-        for (int e = 0; e < t->n_exceptions; ++e) {
-            const int i = t->exception_atoms[2 * e], j = t->exception_atoms[2 * e + 1];
-            expart[i].push_back(j);
-            expart[j].push_back(i);
-        }
-        for (int i = 0; i < H.n_slots; ++i) {
-            if (i < n) {
-                std::vector<int> sk, ex;
-                auto &xp = expart[i];
-                std::sort(xp.begin(), xp.end());
-                for (int e = t->exclusion_offsets[i]; e < t->exclusion_offsets[i + 1]; ++e) {
-                    const int j = t->exclusion_atoms[e];
-                    sk.push_back(j);
-                    if (!std::binary_search(xp.begin(), xp.end(), j)) ex.push_back(j);  // fused_engine.rs:11718-11731
-                }
-                for (int j : xp) sk.push_back(j);
-                std::sort(sk.begin(), sk.end());
-                sk.erase(std::unique(sk.begin(), sk.end()), sk.end());
-                std::sort(ex.begin(), ex.end());
-                ex.erase(std::unique(ex.begin(), ex.end()), ex.end());
-                for (int j : sk) skip_idx.push_back(H.atom_begin + j);
-                for (int j : ex) ex_idx.push_back(H.atom_begin + j);
-            }
-            skip_off.push_back((int)skip_idx.size());
-            ex_off.push_back((int)ex_idx.size());
-        }
-        // constraint groups: clusters, then lone atoms, padded to whole waves
-        std::vector<char> in_cluster(n, 0);
-        std::vector<Group> g;
-        std::vector<GroupParam> gpp;
-        for (int c = 0; c < t->n_clusters; ++c) {
-            Group G{{-1, -1, -1, -1}};
-            GroupParam Q{};
-            const int nh = t->cluster_n_hydrogens[c];
-            G.atom[0] = H.atom_begin + t->cluster_central[c];
-            in_cluster[t->cluster_central[c]] = 1;
-            for (int h = 0; h < nh; ++h) {
-                G.atom[1 + h] = H.atom_begin + t->cluster_hydrogens[3 * c + h];
-                in_cluster[t->cluster_hydrogens[3 * c + h]] = 1;
-                Q.d[h] = t->cluster_bond_lengths[3 * c + h];
-            }
-            Q.n_h = nh;
-            Q.inv_mass_c = t->cluster_inv_mass_central[c];
-            Q.inv_mass_h = t->cluster_inv_mass_h[c];
-            g.push_back(G);
-            gpp.push_back(Q);
-        }
-        for (int i = 0; i < n; ++i)
-            if (!in_cluster[i]) {
-                Group G{{H.atom_begin + i, -1, -1, -1}};
-                GroupParam Q{};
-                Q.inv_mass_c = 1.0f / t->masses[i];
-                g.push_back(G);
-                gpp.push_back(Q);
-            }
-        const int nw = ((int)g.size() + 63) / 64;
-        H.wave_begin = W;
-        H.n_waves = nw;
-        W += nw;
-        while ((int)g.size() < nw * 64) {
-            g.push_back(Group{{-1, -1, -1, -1}});
-            gpp.push_back(GroupParam{});
-        }
-        hgroups.insert(hgroups.end(), g.begin(), g.end());
-        hgp.insert(hgp.end(), gpp.begin(), gpp.end());
-        for (int w = 0; w < nw; ++w) hwave_sys.push_back(s);
-        const int o = H.atom_begin;
-        const int tb0 = (int)hb.size(), ta0 = (int)ha.size(), td0 = (int)hd.size(), te0 = (int)hx14.size();
-        for (int b = 0; b < t->n_bonds; ++b)
-            hb.push_back(Bond{o + t->bond_atoms[2 * b], o + t->bond_atoms[2 * b + 1], t->bond_r0[b], t->bond_k[b]});
-        for (int a = 0; a < t->n_angles; ++a)
-            ha.push_back(Angle{o + t->angle_atoms[3 * a], o + t->angle_atoms[3 * a + 1], o + t->angle_atoms[3 * a + 2],
-                               t->angle_theta0[a], t->angle_k[a], {0, 0, 0}});
-        for (int d = 0; d < t->n_dihedrals; ++d)
-            hd.push_back(Dihedral{o + t->dihedral_atoms[4 * d], o + t->dihedral_atoms[4 * d + 1], o + t->dihedral_atoms[4 * d + 2],
-                                  o + t->dihedral_atoms[4 * d + 3], t->dihedral_periodicity[d], t->dihedral_phase[d],
-                                  t->dihedral_k[d], 0});
-        for (int e = 0; e < t->n_exceptions; ++e)
-            hx14.push_back(Exception14{o + t->exception_atoms[2 * e], o + t->exception_atoms[2 * e + 1],
-                                       t->exception_charge_product[e], t->exception_sigma[e], t->exception_epsilon[e], {0, 0, 0}});
-        // reference cell grid from the creation-time coordinates, float
-        // arithmetic of fused_engine.rs:12285-12330
-        H.x0.assign(t->positions, t->positions + 3 * (size_t)n);
-        float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-        for (int i = 0; i < n; ++i)
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = std::min(lo[c], t->positions[3 * i + c]);
-                hi[c] = std::max(hi[c], t->positions[3 * i + c]);
-            }
-        int dims[3];
-        H.grid_ok = 1;
-        for (int c = 0; c < 3; ++c) {
-            const float extent = (hi[c] - lo[c]) + 2.0f * 10.0f;
-            const long v = std::max(1L, (long)std::ceil(extent / 10.0f));
-            if (v > 32) H.grid_ok = 0;
-            dims[c] = (int)v;
-        }
-        H.grid = RefGrid{lo[0] - 10.0f, lo[1] - 10.0f, lo[2] - 10.0f, dims[0], dims[1], dims[2]};
-        H.dof = std::max(1.0, 3.0 * n - (double)std::accumulate(t->cluster_n_hydrogens, t->cluster_n_hydrogens + t->n_clusters, 0) - 3.0);
-        SysDesc D{};
-        D.atom_begin = H.atom_begin;
-        D.n_atoms = n;
-        D.block_begin = H.block_begin;
-        D.n_blocks = H.n_blocks;
-        D.wave_begin = H.wave_begin;
-        D.n_waves = H.n_waves;
-        D.active = 1;
-        D.rebuild_every = p.rebuild_every;
-        D.com_every = p.com_remove_every;
-        D.constraint_max_iter = p.constraint_max_iter;
-        D.sort_bits = 10;
-        D.resort_every = std::max(1, p.resort_every);
-        D.dt = (float)p.dt_ps;
-        D.cutoff = (float)p.cutoff_A;
-        D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);
-        D.list_cutoff = (float)(p.cutoff_A + p.skin_A);
-        D.half_skin2 = (float)(0.25 * p.skin_A * p.skin_A);
-        D.shake_tol = p.shake_tol;
-        D.rattle_tol = p.rattle_tol;
-        D.inv_total_mass = 1.0 / mt;
-        D.t_start = (float)p.temperature_K;
-        D.t_end = (float)p.end_temperature_K;
-        D.gamma_base = (float)p.gamma_ps;
-        D.gamma_eq = (float)p.equilibration_gamma_ps;
-        D.cold_hold_end = p.cold_hold_end;
-        D.ramp_end = p.ramp_end;
-        D.warm_hold_end = p.warm_hold_end;
-        D.ramp_down_end = p.ramp_down_end;
-        D.warm_tail = p.warm_tail;
-        D.equilibration_steps = p.equilibration_steps;
-        D.seed_lo = (uint32_t)p.seed;
-        D.stream = p.stream_id;
-        D.bond_begin = tb0;
-        D.n_bonds = t->n_bonds;
-        D.angle_begin = ta0;
-        D.n_angles = t->n_angles;
-        D.dih_begin = td0;
-        D.n_dih = t->n_dihedrals;
-        D.exc_begin = te0;
-        D.n_exc = t->n_exceptions;
-        desc_.push_back(D);
-        sys_.push_back(std::move(H));
+    size_t NB = 0, NA = 0, ND = 0, NE = 0, SK = 0, EX = 0, CS = 0;
+    slots_.clear();
+    for (const auto &c : caps) {
+        if (c.atoms < 2 || c.bonds < 0 || c.angles < 0 || c.dihedrals < 0 || c.exceptions < 0 || c.exclusions < 0)
+            return fail(PRISM_MD_E_INVALID, "slot capacities");
+        Slot S{};
+        S.atom_base = A;
+        S.atom_cap = pad64(c.atoms);
+        S.wave_base = W;
+        S.wave_cap = S.atom_cap / 64;
+        S.bond_base = (int)NB; S.bond_cap = c.bonds;
+        S.angle_base = (int)NA; S.angle_cap = c.angles;
+        S.dih_base = (int)ND; S.dih_cap = c.dihedrals;
+        S.exc_base = (int)NE; S.exc_cap = c.exceptions;
+        S.skip_base = (int)SK; S.skip_cap = c.exclusions + 2 * c.exceptions;
+        S.ex_base = (int)EX; S.ex_cap = c.exclusions;
+        S.contrib_base = (int)CS;
+        S.contrib_cap = 2 * c.bonds + 3 * c.angles + 4 * c.dihedrals + 2 * c.exceptions;
+        S.caps = c;
+        A += S.atom_cap;
+        W += S.wave_cap;
+        NB += S.bond_cap; NA += S.angle_cap; ND += S.dih_cap; NE += S.exc_cap;
+        SK += S.skip_cap; EX += S.ex_cap; CS += S.contrib_cap;
+        slots_.push_back(S);
     }
     const int B = A / BLOCK;
-    // ---- bonded contribution slots, atom-major: the slots of one atom are
-    // contiguous and ordered by term kind, term index and position, so the
-    // gather adds every atom's bonded forces in one fixed order
-    std::vector<int> inc_off(A + 1, 0);
-    auto count_atoms = [&](const int *atoms, size_t n_terms, int arity, size_t stride) {
-        for (size_t k = 0; k < n_terms; ++k)
-            for (int p = 0; p < arity; ++p) ++inc_off[atoms[k * stride + p] + 1];
-    };
-    count_atoms(hb.empty() ? nullptr : &hb[0].i, hb.size(), 2, sizeof(Bond) / sizeof(int));
-    count_atoms(ha.empty() ? nullptr : &ha[0].i, ha.size(), 3, sizeof(Angle) / sizeof(int));
-    count_atoms(hd.empty() ? nullptr : &hd[0].i, hd.size(), 4, sizeof(Dihedral) / sizeof(int));
-    count_atoms(hx14.empty() ? nullptr : &hx14[0].i, hx14.size(), 2, sizeof(Exception14) / sizeof(int));
-    for (int a = 0; a < A; ++a) inc_off[a + 1] += inc_off[a];
-    std::vector<int> fill(inc_off.begin(), inc_off.end() - 1);
-    auto assign = [&](const int *atoms, size_t n_terms, int arity, size_t stride, std::vector<int> &dst) {
-        dst.resize(n_terms * arity);
-        for (size_t k = 0; k < n_terms; ++k)
-            for (int p = 0; p < arity; ++p) dst[k * arity + p] = fill[atoms[k * stride + p]]++;
-    };
-    std::vector<int> dst_b, dst_a, dst_d, dst_e;
-    assign(hb.empty() ? nullptr : &hb[0].i, hb.size(), 2, sizeof(Bond) / sizeof(int), dst_b);
-    assign(ha.empty() ? nullptr : &ha[0].i, ha.size(), 3, sizeof(Angle) / sizeof(int), dst_a);
-    assign(hd.empty() ? nullptr : &hd[0].i, hd.size(), 4, sizeof(Dihedral) / sizeof(int), dst_d);
-    assign(hx14.empty() ? nullptr : &hx14[0].i, hx14.size(), 2, sizeof(Exception14) / sizeof(int), dst_e);
-    const size_t n_slots = (size_t)inc_off[A];
-    const size_t n_terms = hb.size() + ha.size() + hd.size() + hx14.size();
+    const int n_sys = (int)caps.size();
+    // host mirror, all padding
+    HostArena &h = h_;
+    h.x.assign(A, make_float4(0, 0, 0, 0));
+    h.v.assign(A, make_float4(0, 0, 0, 0));
+    h.mass.assign(A, 1.0f);
+    h.q.assign(A, 0.0f);
+    h.sig.assign(A, 0.0f);
+    h.eps.assign(A, 0.0f);
+    h.atom_sys.assign(A, -1);
+    h.block_sys.assign(B, -1);
+    h.wave_sys.assign(W, -1);
+    h.skip_off.assign(A + 1, 0);
+    h.ex_off.assign(A + 1, 0);
+    h.inc_off.assign(A + 1, 0);
+    for (const auto &S : slots_)
+        for (int a = S.atom_base; a <= S.atom_base + S.atom_cap && a <= A; ++a) {
+            h.skip_off[a] = S.skip_base;
+            h.ex_off[a] = S.ex_base;
+            h.inc_off[a] = S.contrib_base;
+        }
+    h.skip_idx.assign(std::max<size_t>(SK, 1), 0);
+    h.ex_idx.assign(std::max<size_t>(EX, 1), 0);
+    h.groups.assign((size_t)W * 64, Group{{-1, -1, -1, -1}});
+    h.gp.assign((size_t)W * 64, GroupParam{});
+    h.bonds.assign(NB, Bond{-1, -1, 0.0f, 0.0f});
+    h.angles.assign(NA, Angle{-1, -1, -1, 0.0f, 0.0f, {0, 0, 0}});
+    h.dih.assign(ND, Dihedral{-1, -1, -1, -1, 0, 0.0f, 0.0f, 0});
+    h.exc.assign(NE, Exception14{-1, -1, 0.0f, 0.0f, 0.0f, {0, 0, 0}});
+    h.dst_b.assign(2 * NB, 0);
+    h.dst_a.assign(3 * NA, 0);
+    h.dst_d.assign(4 * ND, 0);
+    h.dst_e.assign(2 * NE, 0);
+    sys_.assign(n_sys, HostSys{});
+    desc_.assign(n_sys, SysDesc{});
+    for (int k = 0; k < n_sys; ++k) {
+        desc_[k].atom_begin = slots_[k].atom_base;
+        desc_[k].block_begin = slots_[k].atom_base / BLOCK;
+        desc_[k].wave_begin = slots_[k].wave_base;
+        desc_[k].bond_begin = slots_[k].bond_base;
+        desc_[k].angle_begin = slots_[k].angle_base;
+        desc_[k].dih_begin = slots_[k].dih_base;
+        desc_[k].exc_begin = slots_[k].exc_base;
+        desc_[k].resort_every = 1;
+        desc_[k].inv_total_mass = 1.0;
+    }
+    const size_t n_terms = NB + NA + ND + NE;
     // ---- device allocation
-    P_.n_sys = (int)systems.size();
+    P_.n_sys = n_sys;
     P_.n_atoms_arena = A;
     P_.n_blocks_arena = B;
     P_.n_waves_arena = W;
@@ -388,15 +268,15 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
     P_.xs = dalloc<float4>(A);
     P_.x_build = dalloc<float4>(A);
     P_.f = dalloc<long long>((size_t)A * 4);
-    float *d_mass = dalloc<float>(A), *d_q = dalloc<float>(A), *d_sig = dalloc<float>(A), *d_eps = dalloc<float>(A);
-    int *d_atom_sys = dalloc<int>(A);
+    d_mass_ = dalloc<float>(A); d_q_ = dalloc<float>(A); d_sig_ = dalloc<float>(A); d_eps_ = dalloc<float>(A);
+    d_atom_sys_ = dalloc<int>(A);
     P_.perm = dalloc<int>(A);
     P_.slot_of = dalloc<int>(A);
     P_.xq = dalloc<float4>(A);
     P_.lj = dalloc<float2>(A);
     P_.bb_c = dalloc<float4>(B);
     P_.bb_h = dalloc<float4>(B);
-    int *d_block_sys = dalloc<int>(B);
+    d_block_sys_ = dalloc<int>(B);
     P_.row_n = dalloc<int>(B);
     P_.row_j = dalloc<int>((size_t)B * MAX_ROW);
     P_.row_m = dalloc<int>((size_t)B * MAX_ROW);
@@ -407,23 +287,23 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
     P_.units = dalloc<int>((size_t)B * UNITS_PER_BLOCK);
     P_.unit_off = dalloc<int>(B);
     P_.unit_f = dalloc<float4>((size_t)B * UNITS_PER_BLOCK * SUB);
-    P_.contrib = dalloc<float>(n_slots * 3);
-    int *d_inc_off = dalloc<int>(inc_off.size());
-    int *d_dst_b = dalloc<int>(dst_b.size()), *d_dst_a = dalloc<int>(dst_a.size());
-    int *d_dst_d = dalloc<int>(dst_d.size()), *d_dst_e = dalloc<int>(dst_e.size());
+    P_.contrib = dalloc<float>(std::max<size_t>(CS, 1) * 3);
+    d_inc_off_ = dalloc<int>(A + 1);
+    d_dst_b_ = dalloc<int>(2 * NB); d_dst_a_ = dalloc<int>(3 * NA);
+    d_dst_d_ = dalloc<int>(4 * ND); d_dst_e_ = dalloc<int>(2 * NE);
     P_.term_e = dalloc<double>(n_terms);
     P_.atom_e = dalloc<double>(A);
-    int *d_skip_off = dalloc<int>(skip_off.size()), *d_skip_idx = dalloc<int>(skip_idx.size());
-    int *d_ex_off = dalloc<int>(ex_off.size()), *d_ex_idx = dalloc<int>(ex_idx.size());
-    Group *d_groups = dalloc<Group>((size_t)W * 64);
-    GroupParam *d_gp = dalloc<GroupParam>((size_t)W * 64);
-    int *d_wave_sys = dalloc<int>(W);
-    Bond *d_b = dalloc<Bond>(hb.size());
-    Angle *d_a = dalloc<Angle>(ha.size());
-    Dihedral *d_d = dalloc<Dihedral>(hd.size());
-    Exception14 *d_e = dalloc<Exception14>(hx14.size());
-    d_desc_ = dalloc<SysDesc>(desc_.size());
-    P_.sst = dalloc<SysState>(desc_.size());
+    d_skip_off_ = dalloc<int>(A + 1); d_skip_idx_ = dalloc<int>(h.skip_idx.size());
+    d_ex_off_ = dalloc<int>(A + 1); d_ex_idx_ = dalloc<int>(h.ex_idx.size());
+    d_groups_ = dalloc<Group>((size_t)W * 64);
+    d_gp_ = dalloc<GroupParam>((size_t)W * 64);
+    d_wave_sys_ = dalloc<int>(W);
+    d_bonds_ = dalloc<Bond>(NB);
+    d_angles_ = dalloc<Angle>(NA);
+    d_dih_ = dalloc<Dihedral>(ND);
+    d_exc_ = dalloc<Exception14>(NE);
+    d_desc_ = dalloc<SysDesc>(n_sys);
+    P_.sst = dalloc<SysState>(n_sys);
     P_.keys_a = dalloc<unsigned int>(A);
     P_.keys_b = dalloc<unsigned int>(A);
     P_.vals_a = dalloc<int>(A);
@@ -431,40 +311,342 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
     P_.com_partial = dalloc<double>((size_t)W * 3);
     P_.counters = dalloc<EngineCounters>(1);
     d_count_ = dalloc<unsigned long long>(1);
-    d_red_ = dalloc<double>(desc_.size() * 4);
+    d_red_ = dalloc<double>((size_t)n_sys * 4);
     d_noise_ = dalloc<float>((size_t)2 * A * 3);
     d_vmin_ = dalloc<float4>(A);
     d_fire_ = dalloc<double>((size_t)(A / 256 + 1) * 4);
     if (alloc_failed_) return fail(PRISM_MD_E_NOMEM, "device allocation failed");
-    P_.mass = d_mass; P_.charge = d_q; P_.sigma = d_sig; P_.eps = d_eps; P_.atom_sys = d_atom_sys;
-    P_.block_sys = d_block_sys;
-    P_.skip_off = d_skip_off; P_.skip_idx = d_skip_idx; P_.ex_off = d_ex_off; P_.ex_idx = d_ex_idx;
-    P_.groups = d_groups; P_.gparams = d_gp; P_.wave_sys = d_wave_sys;
-    P_.bonds = d_b; P_.angles = d_a; P_.dih = d_d; P_.exc = d_e;
-    P_.inc_off = d_inc_off;
-    P_.dst_bond = d_dst_b; P_.dst_angle = d_dst_a; P_.dst_dih = d_dst_d; P_.dst_exc = d_dst_e;
-    P_.n_bonds = (int)hb.size(); P_.n_angles = (int)ha.size(); P_.n_dih = (int)hd.size(); P_.n_exc = (int)hx14.size();
+    P_.mass = d_mass_; P_.charge = d_q_; P_.sigma = d_sig_; P_.eps = d_eps_; P_.atom_sys = d_atom_sys_;
+    P_.block_sys = d_block_sys_;
+    P_.skip_off = d_skip_off_; P_.skip_idx = d_skip_idx_; P_.ex_off = d_ex_off_; P_.ex_idx = d_ex_idx_;
+    P_.groups = d_groups_; P_.gparams = d_gp_; P_.wave_sys = d_wave_sys_;
+    P_.bonds = d_bonds_; P_.angles = d_angles_; P_.dih = d_dih_; P_.exc = d_exc_;
+    P_.inc_off = d_inc_off_;
+    P_.dst_bond = d_dst_b_; P_.dst_angle = d_dst_a_; P_.dst_dih = d_dst_d_; P_.dst_exc = d_dst_e_;
+    P_.n_bonds = (int)NB; P_.n_angles = (int)NA; P_.n_dih = (int)ND; P_.n_exc = (int)NE;
     P_.sys = d_desc_;
+    const std::vector<int> minus1(A, -1);
     int rc = PRISM_MD_OK;
-    if ((rc = upload(P_.x, hx)) || (rc = upload(P_.v, hv)) || (rc = upload(d_mass, hmass)) || (rc = upload(d_q, hq)) ||
-        (rc = upload(d_sig, hsig)) || (rc = upload(d_eps, heps)) || (rc = upload(d_atom_sys, hatom_sys)) ||
-        (rc = upload(d_block_sys, hblock_sys)) || (rc = upload(d_skip_off, skip_off)) || (rc = upload(d_skip_idx, skip_idx)) ||
-        (rc = upload(d_ex_off, ex_off)) || (rc = upload(d_ex_idx, ex_idx)) || (rc = upload(d_groups, hgroups)) ||
-        (rc = upload(d_gp, hgp)) || (rc = upload(d_wave_sys, hwave_sys)) || (rc = upload(d_b, hb)) || (rc = upload(d_a, ha)) ||
-        (rc = upload(d_d, hd)) || (rc = upload(d_e, hx14)) || (rc = upload(d_desc_, desc_)) ||
-        (rc = upload(d_inc_off, inc_off)) || (rc = upload(d_dst_b, dst_b)) || (rc = upload(d_dst_a, dst_a)) ||
-        (rc = upload(d_dst_d, dst_d)) || (rc = upload(d_dst_e, dst_e)))
+    if ((rc = upload(P_.x, h.x)) || (rc = upload(P_.v, h.v)) || (rc = upload(d_mass_, h.mass)) ||
+        (rc = upload(d_q_, h.q)) || (rc = upload(d_sig_, h.sig)) || (rc = upload(d_eps_, h.eps)) ||
+        (rc = upload(d_atom_sys_, h.atom_sys)) || (rc = upload(d_block_sys_, h.block_sys)) ||
+        (rc = upload(d_skip_off_, h.skip_off)) || (rc = upload(d_skip_idx_, h.skip_idx)) ||
+        (rc = upload(d_ex_off_, h.ex_off)) || (rc = upload(d_ex_idx_, h.ex_idx)) || (rc = upload(d_groups_, h.groups)) ||
+        (rc = upload(d_gp_, h.gp)) || (rc = upload(d_wave_sys_, h.wave_sys)) || (rc = upload(d_bonds_, h.bonds)) ||
+        (rc = upload(d_angles_, h.angles)) || (rc = upload(d_dih_, h.dih)) || (rc = upload(d_exc_, h.exc)) ||
+        (rc = upload(d_desc_, desc_)) || (rc = upload(d_inc_off_, h.inc_off)) || (rc = upload(d_dst_b_, h.dst_b)) ||
+        (rc = upload(d_dst_a_, h.dst_a)) || (rc = upload(d_dst_d_, h.dst_d)) || (rc = upload(d_dst_e_, h.dst_e)) ||
+        (rc = upload(P_.perm, minus1)))
         return rc;
-    std::vector<SysState> st(desc_.size());
+    std::vector<SysState> st(n_sys);
     for (auto &S : st) {
         std::memset(&S, 0, sizeof(S));
         S.rebuild_step = -1;
         S.fault_atom = -1;
     }
-    if ((rc = upload(P_.sst, st))) return rc;
+    return upload(P_.sst, st);
+}
+
+// Lower one topology into slot s of the host mirror (padding after it) and
+// fill the slot's descriptor.  Nothing is uploaded here.
+int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
+    const Slot &L = slots_[s];
+    const std::string over = fits(L.caps, t);
+    if (!over.empty()) return fail(PRISM_MD_E_CAPACITY, "system does not fit its slot: " + over);
+    HostArena &h = h_;
+    const int n = t->n_atoms;
+    const int o = L.atom_base;
+    HostSys H{};
+    H.atom_begin = o;
+    H.n_atoms = n;
+    H.n_slots = L.atom_cap;
+    H.block_begin = o / BLOCK;
+    H.n_blocks = (n + BLOCK - 1) / BLOCK;
+    std::vector<float> v0;
+    maxwell_boltzmann(t, p, v0);
+    double mt = 0.0;
+    for (int i = 0; i < L.atom_cap; ++i) {
+        const int a = o + i;
+        if (i < n) {
+            h.x[a] = make_float4(t->positions[3 * i], t->positions[3 * i + 1], t->positions[3 * i + 2], 0.0f);
+            h.v[a] = make_float4(v0[3 * i], v0[3 * i + 1], v0[3 * i + 2], 0.0f);
+            h.mass[a] = t->masses[i];
+            h.q[a] = t->charges[i];
+            h.sig[a] = t->sigma[i];
+            h.eps[a] = t->epsilon[i];
+            h.atom_sys[a] = s;
+            mt += t->masses[i];
+        } else {
+            h.x[a] = h.v[a] = make_float4(0, 0, 0, 0);
+            h.mass[a] = 1.0f;
+            h.q[a] = h.sig[a] = h.eps[a] = 0.0f;
+            h.atom_sys[a] = -1;
+        }
+    }
+    for (int b = 0; b < L.atom_cap / BLOCK; ++b) h.block_sys[H.block_begin + b] = b < H.n_blocks ? s : -1;
+    // skip lists (exclusions U exception partners) and pair-list exclusions
+    // (exclusions minus exception partners, fused_engine.rs:11718-11731)
+    std::vector<std::vector<int>> expart(n);
+    for (int e = 0; e < t->n_exceptions; ++e) {
+        const int i = t->exception_atoms[2 * e], j = t->exception_atoms[2 * e + 1];
+        expart[i].push_back(j);
+        expart[j].push_back(i);
+    }
+    int sk_n = 0, ex_n = 0;
+    for (int i = 0; i < L.atom_cap; ++i) {
+        h.skip_off[o + i] = L.skip_base + sk_n;
+        h.ex_off[o + i] = L.ex_base + ex_n;
+        if (i >= n) continue;
+        std::vector<int> sk, ex;
+        auto &xp = expart[i];
+        std::sort(xp.begin(), xp.end());
+        for (int e = t->exclusion_offsets[i]; e < t->exclusion_offsets[i + 1]; ++e) {
+            const int j = t->exclusion_atoms[e];
+            sk.push_back(j);
+            if (!std::binary_search(xp.begin(), xp.end(), j)) ex.push_back(j);
+        }
+        for (int j : xp) sk.push_back(j);
+        std::sort(sk.begin(), sk.end());
+        sk.erase(std::unique(sk.begin(), sk.end()), sk.end());
+        std::sort(ex.begin(), ex.end());
+        ex.erase(std::unique(ex.begin(), ex.end()), ex.end());
+        if (sk_n + (int)sk.size() > L.skip_cap || ex_n + (int)ex.size() > L.ex_cap)
+            return fail(PRISM_MD_E_CAPACITY, "system does not fit its slot: exclusion lists");
+        for (int j : sk) h.skip_idx[L.skip_base + sk_n++] = o + j;
+        for (int j : ex) h.ex_idx[L.ex_base + ex_n++] = o + j;
+    }
+    // constraint groups: clusters, then lone atoms, then padding
+    std::vector<char> in_cluster(n, 0);
+    int ng = 0;
+    auto put_group = [&](const Group &G, const GroupParam &Q) {
+        h.groups[(size_t)L.wave_base * 64 + ng] = G;
+        h.gp[(size_t)L.wave_base * 64 + ng] = Q;
+        ++ng;
+    };
+    for (int c = 0; c < t->n_clusters; ++c) {
+        Group G{{-1, -1, -1, -1}};
+        GroupParam Q{};
+        const int nh = t->cluster_n_hydrogens[c];
+        G.atom[0] = o + t->cluster_central[c];
+        in_cluster[t->cluster_central[c]] = 1;
+        for (int k = 0; k < nh; ++k) {
+            G.atom[1 + k] = o + t->cluster_hydrogens[3 * c + k];
+            in_cluster[t->cluster_hydrogens[3 * c + k]] = 1;
+            Q.d[k] = t->cluster_bond_lengths[3 * c + k];
+        }
+        Q.n_h = nh;
+        Q.inv_mass_c = t->cluster_inv_mass_central[c];
+        Q.inv_mass_h = t->cluster_inv_mass_h[c];
+        put_group(G, Q);
+    }
+    for (int i = 0; i < n; ++i)
+        if (!in_cluster[i]) {
+            GroupParam Q{};
+            Q.inv_mass_c = 1.0f / t->masses[i];
+            put_group(Group{{o + i, -1, -1, -1}}, Q);
+        }
+    H.wave_begin = L.wave_base;
+    H.n_waves = (ng + 63) / 64;
+    while (ng < L.wave_cap * 64) put_group(Group{{-1, -1, -1, -1}}, GroupParam{});
+    for (int w = 0; w < L.wave_cap; ++w) h.wave_sys[L.wave_base + w] = w < H.n_waves ? s : -1;
+    // terms (padding records have i = -1)
+    for (int b = 0; b < L.bond_cap; ++b)
+        h.bonds[L.bond_base + b] = b < t->n_bonds
+            ? Bond{o + t->bond_atoms[2 * b], o + t->bond_atoms[2 * b + 1], t->bond_r0[b], t->bond_k[b]}
+            : Bond{-1, -1, 0.0f, 0.0f};
+    for (int a = 0; a < L.angle_cap; ++a)
+        h.angles[L.angle_base + a] = a < t->n_angles
+            ? Angle{o + t->angle_atoms[3 * a], o + t->angle_atoms[3 * a + 1], o + t->angle_atoms[3 * a + 2],
+                    t->angle_theta0[a], t->angle_k[a], {0, 0, 0}}
+            : Angle{-1, -1, -1, 0.0f, 0.0f, {0, 0, 0}};
+    for (int d = 0; d < L.dih_cap; ++d)
+        h.dih[L.dih_base + d] = d < t->n_dihedrals
+            ? Dihedral{o + t->dihedral_atoms[4 * d], o + t->dihedral_atoms[4 * d + 1], o + t->dihedral_atoms[4 * d + 2],
+                       o + t->dihedral_atoms[4 * d + 3], t->dihedral_periodicity[d], t->dihedral_phase[d],
+                       t->dihedral_k[d], 0}
+            : Dihedral{-1, -1, -1, -1, 0, 0.0f, 0.0f, 0};
+    for (int e = 0; e < L.exc_cap; ++e)
+        h.exc[L.exc_base + e] = e < t->n_exceptions
+            ? Exception14{o + t->exception_atoms[2 * e], o + t->exception_atoms[2 * e + 1],
+                          t->exception_charge_product[e], t->exception_sigma[e], t->exception_epsilon[e], {0, 0, 0}}
+            : Exception14{-1, -1, 0.0f, 0.0f, 0.0f, {0, 0, 0}};
+    // bonded contribution slots, atom-major: the slots of one atom are
+    // contiguous and ordered by term kind, term index and position, so the
+    // gather adds every atom's bonded forces in one fixed order
+    std::vector<int> cnt(L.atom_cap + 1, 0);
+    auto count_atoms = [&](const int *atoms, int n_terms, int arity) {
+        for (int k = 0; k < n_terms; ++k)
+            for (int q = 0; q < arity; ++q) ++cnt[atoms[k * arity + q] + 1];
+    };
+    count_atoms(t->bond_atoms, t->n_bonds, 2);
+    count_atoms(t->angle_atoms, t->n_angles, 3);
+    count_atoms(t->dihedral_atoms, t->n_dihedrals, 4);
+    count_atoms(t->exception_atoms, t->n_exceptions, 2);
+    for (int i = 0; i < L.atom_cap; ++i) cnt[i + 1] += cnt[i];
+    for (int i = 0; i < L.atom_cap; ++i) h.inc_off[o + i] = L.contrib_base + cnt[i];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    auto assign = [&](const int *atoms, int n_terms, int arity, std::vector<int> &dst, int base) {
+        for (int k = 0; k < n_terms; ++k)
+            for (int q = 0; q < arity; ++q) dst[(size_t)(base + k) * arity + q] = L.contrib_base + fill[atoms[k * arity + q]]++;
+    };
+    assign(t->bond_atoms, t->n_bonds, 2, h.dst_b, L.bond_base);
+    assign(t->angle_atoms, t->n_angles, 3, h.dst_a, L.angle_base);
+    assign(t->dihedral_atoms, t->n_dihedrals, 4, h.dst_d, L.dih_base);
+    assign(t->exception_atoms, t->n_exceptions, 2, h.dst_e, L.exc_base);
+    // reference cell grid from the creation-time coordinates, float
+    // arithmetic of fused_engine.rs:12285-12330
+    H.x0.assign(t->positions, t->positions + 3 * (size_t)n);
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], t->positions[3 * i + c]);
+            hi[c] = std::max(hi[c], t->positions[3 * i + c]);
+        }
+    int dims[3];
+    H.grid_ok = 1;
+    for (int c = 0; c < 3; ++c) {
+        const float extent = (hi[c] - lo[c]) + 2.0f * 10.0f;
+        const long v = std::max(1L, (long)std::ceil(extent / 10.0f));
+        if (v > 32) H.grid_ok = 0;
+        dims[c] = (int)v;
+    }
+    H.grid = RefGrid{lo[0] - 10.0f, lo[1] - 10.0f, lo[2] - 10.0f, dims[0], dims[1], dims[2]};
+    H.dof = std::max(1.0, 3.0 * n - (double)std::accumulate(t->cluster_n_hydrogens, t->cluster_n_hydrogens + t->n_clusters, 0) - 3.0);
+    SysDesc D{};
+    D.atom_begin = H.atom_begin;
+    D.n_atoms = n;
+    D.block_begin = H.block_begin;
+    D.n_blocks = H.n_blocks;
+    D.wave_begin = H.wave_begin;
+    D.n_waves = H.n_waves;
+    D.active = 1;
+    D.rebuild_every = p.rebuild_every;
+    D.com_every = p.com_remove_every;
+    D.constraint_max_iter = p.constraint_max_iter;
+    D.sort_bits = 10;
+    D.resort_every = std::max(1, p.resort_every);
+    D.dt = (float)p.dt_ps;
+    D.cutoff = (float)p.cutoff_A;
+    D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);
+    D.list_cutoff = (float)(p.cutoff_A + p.skin_A);
+    D.half_skin2 = (float)(0.25 * p.skin_A * p.skin_A);
+    D.shake_tol = p.shake_tol;
+    D.rattle_tol = p.rattle_tol;
+    D.inv_total_mass = 1.0 / mt;
+    D.t_start = (float)p.temperature_K;
+    D.t_end = (float)p.end_temperature_K;
+    D.gamma_base = (float)p.gamma_ps;
+    D.gamma_eq = (float)p.equilibration_gamma_ps;
+    D.cold_hold_end = p.cold_hold_end;
+    D.ramp_end = p.ramp_end;
+    D.warm_hold_end = p.warm_hold_end;
+    D.ramp_down_end = p.ramp_down_end;
+    D.warm_tail = p.warm_tail;
+    D.equilibration_steps = p.equilibration_steps;
+    D.seed_lo = (uint32_t)p.seed;
+    D.stream = p.stream_id;
+    D.bond_begin = L.bond_base;
+    D.n_bonds = t->n_bonds;
+    D.angle_begin = L.angle_base;
+    D.n_angles = t->n_angles;
+    D.dih_begin = L.dih_base;
+    D.n_dih = t->n_dihedrals;
+    D.exc_begin = L.exc_base;
+    D.n_exc = t->n_exceptions;
+    desc_[s] = D;
+    sys_[s] = std::move(H);
+    return PRISM_MD_OK;
+}
+
+template <class T> int Engine::upload_range(T *d, const std::vector<T> &h, size_t begin, size_t count) {
+    if (count == 0) return PRISM_MD_OK;
+    return hipcheck(hipMemcpyAsync(d + begin, h.data() + begin, count * sizeof(T), hipMemcpyHostToDevice, stream_),
+                    "slot upload");
+}
+
+// Upload slot s of the host mirror and its descriptor; reset its state.
+int Engine::upload_slot(int s) {
+    const Slot &L = slots_[s];
+    const HostArena &h = h_;
+    const size_t a0 = L.atom_base, na = L.atom_cap, b0 = a0 / BLOCK, nb = na / BLOCK;
+    const size_t g0 = (size_t)L.wave_base * 64, ng = (size_t)L.wave_cap * 64;
+    int rc = PRISM_MD_OK;
+    if ((rc = upload_range(P_.x, h.x, a0, na)) || (rc = upload_range(P_.v, h.v, a0, na)) ||
+        (rc = upload_range(d_mass_, h.mass, a0, na)) || (rc = upload_range(d_q_, h.q, a0, na)) ||
+        (rc = upload_range(d_sig_, h.sig, a0, na)) || (rc = upload_range(d_eps_, h.eps, a0, na)) ||
+        (rc = upload_range(d_atom_sys_, h.atom_sys, a0, na)) || (rc = upload_range(d_block_sys_, h.block_sys, b0, nb)) ||
+        (rc = upload_range(d_skip_off_, h.skip_off, a0, na)) || (rc = upload_range(d_ex_off_, h.ex_off, a0, na)) ||
+        (rc = upload_range(d_inc_off_, h.inc_off, a0, na)) ||
+        (rc = upload_range(d_skip_idx_, h.skip_idx, L.skip_base, L.skip_cap)) ||
+        (rc = upload_range(d_ex_idx_, h.ex_idx, L.ex_base, L.ex_cap)) || (rc = upload_range(d_groups_, h.groups, g0, ng)) ||
+        (rc = upload_range(d_gp_, h.gp, g0, ng)) || (rc = upload_range(d_wave_sys_, h.wave_sys, L.wave_base, L.wave_cap)) ||
+        (rc = upload_range(d_bonds_, h.bonds, L.bond_base, L.bond_cap)) ||
+        (rc = upload_range(d_angles_, h.angles, L.angle_base, L.angle_cap)) ||
+        (rc = upload_range(d_dih_, h.dih, L.dih_base, L.dih_cap)) ||
+        (rc = upload_range(d_exc_, h.exc, L.exc_base, L.exc_cap)) ||
+        (rc = upload_range(d_dst_b_, h.dst_b, 2 * (size_t)L.bond_base, 2 * (size_t)L.bond_cap)) ||
+        (rc = upload_range(d_dst_a_, h.dst_a, 3 * (size_t)L.angle_base, 3 * (size_t)L.angle_cap)) ||
+        (rc = upload_range(d_dst_d_, h.dst_d, 4 * (size_t)L.dih_base, 4 * (size_t)L.dih_cap)) ||
+        (rc = upload_range(d_dst_e_, h.dst_e, 2 * (size_t)L.exc_base, 2 * (size_t)L.exc_cap)) ||
+        (rc = upload_range(d_desc_, desc_, (size_t)s, 1)))
+        return rc;
+    // no stale slot permutation: every slot of the range maps to no atom until re-sorted
+    const std::vector<int> minus1(na, -1);
+    if ((rc = hipcheck(hipMemcpyAsync(P_.perm + a0, minus1.data(), na * sizeof(int), hipMemcpyHostToDevice, stream_),
+                       "slot perm")))
+        return rc;
+    SysState S;
+    std::memset(&S, 0, sizeof(S));
+    S.rebuild_step = -1;
+    S.fault_atom = -1;
+    if ((rc = hipcheck(hipMemcpyAsync(P_.sst + s, &S, sizeof(S), hipMemcpyHostToDevice, stream_), "slot state")))
+        return rc;
+    // the host staging buffers above must outlive the copies
+    return hipcheck(hipStreamSynchronize(stream_), "slot upload");
+}
+
+int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
+    std::vector<PrismMdSlotCapsV1> caps;
+    for (const auto &sp : systems) {
+        std::string why;
+        if (validate(sp.topo, sp.params, why) != PRISM_MD_OK) return fail(PRISM_MD_E_INVALID, "invalid topology/params: " + why);
+        caps.push_back(exact_caps(sp.topo));
+    }
+    if (int rc = layout(caps, stream)) return rc;
+    for (int s = 0; s < (int)systems.size(); ++s) {
+        if (int rc = lower(s, systems[s].topo, systems[s].params)) return rc;
+        if (int rc = upload_slot(s)) return rc;
+    }
     launch_kinetic(P_, d_red_, -2, stream_);  // initial COM velocity (zero by construction)
-    if ((rc = refresh_all())) return rc;
+    if (int rc = refresh_all()) return rc;
     return check_status();
+}
+
+int Engine::init_pool(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
+    if (caps.empty()) return fail(PRISM_MD_E_INVALID, "no slots");
+    if (int rc = layout(caps, stream)) return rc;
+    return check_status();
+}
+
+// Admit a system into a free slot: lower, upload the slot, rebuild only it.
+int Engine::admit(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
+    if (s < 0 || s >= (int)slots_.size()) return fail(PRISM_MD_E_INVALID, "slot index");
+    if (desc_[s].active) return fail(PRISM_MD_E_INVALID, "slot is occupied");
+    std::string why;
+    if (validate(t, p, why) != PRISM_MD_OK) return fail(PRISM_MD_E_INVALID, "invalid topology/params: " + why);
+    if (int rc = lower(s, t, p)) {
+        desc_[s].active = 0;
+        return rc;
+    }
+    if (int rc = upload_slot(s)) return rc;
+    launch_kinetic(P_, d_red_, s, stream_);
+    if (int rc = refresh(s)) return rc;
+    return check_status();
+}
+
+int Engine::retire(int s) {
+    if (s < 0 || s >= (int)slots_.size()) return fail(PRISM_MD_E_INVALID, "slot index");
+    return set_active(s, false);
 }
 
 // Rebuild the lists at the current positions and recompute the carried force.
@@ -901,6 +1083,7 @@ int Engine::stats(int s, PrismMdStatsV1 *out) {
 
 int Engine::set_active(int s, bool active) {
     if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    if (sys_[s].n_atoms == 0) return fail(PRISM_MD_E_INVALID, "slot holds no system");
     if ((desc_[s].active != 0) == active) return PRISM_MD_OK;
     desc_[s].active = active ? 1 : 0;
     if (int rc = hipcheck(hipMemcpyAsync(d_desc_ + s, &desc_[s], sizeof(SysDesc), hipMemcpyHostToDevice, stream_),
@@ -1046,7 +1229,8 @@ int prism_md_batch_destroy(PrismMdBatch *b) {
 int prism_md_batch_size(PrismMdBatch *b, int32_t *n_systems, int64_t *n_atoms) {
     if (!b) return PRISM_MD_E_INVALID;
     int64_t total = 0;
-    for (int s = 0; s < b->eng.n_systems(); ++s) total += b->eng.n_atoms(s);
+    for (int s = 0; s < b->eng.n_systems(); ++s)
+        if (b->eng.occupied(s)) total += b->eng.n_atoms(s);
     if (n_systems) *n_systems = b->eng.n_systems();
     if (n_atoms) *n_atoms = total;
     return PRISM_MD_OK;
@@ -1087,6 +1271,46 @@ int prism_md_batch_get_stats(PrismMdBatch *b, int32_t system, PrismMdStatsV1 *st
 }
 
 const char *prism_md_batch_last_error(PrismMdBatch *b) { return b ? b->eng.last_error() : g_create_error.c_str(); }
+
+int prism_md_slot_caps_for(const PrismMdTopologyV1 *t, PrismMdSlotCapsV1 *caps) {
+    if (!t || !caps || t->n_atoms < 1 || !t->exclusion_offsets) return PRISM_MD_E_INVALID;
+    *caps = prism::exact_caps(t);
+    return PRISM_MD_OK;
+}
+
+int prism_md_pool_create(const PrismMdSlotCapsV1 *caps, int32_t n_slots, void *hip_stream, PrismMdBatch **out) {
+    if (!caps || !out || n_slots < 1) return PRISM_MD_E_INVALID;
+    *out = nullptr;
+    PrismMdBatch *b = new (std::nothrow) PrismMdBatch();
+    if (!b) return PRISM_MD_E_NOMEM;
+    const int rc = b->eng.init_pool(std::vector<PrismMdSlotCapsV1>(caps, caps + n_slots), hip_stream);
+    if (rc != PRISM_MD_OK) {
+        g_create_error = b->eng.last_error();
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return PRISM_MD_OK;
+}
+
+int prism_md_batch_admit(PrismMdBatch *b, int32_t slot, const PrismMdTopologyV1 *t, const PrismMdParamsV1 *p) {
+    return b && t && p ? b->eng.admit(slot, t, *p) : PRISM_MD_E_INVALID;
+}
+
+int prism_md_batch_retire(PrismMdBatch *b, int32_t slot) { return b ? b->eng.retire(slot) : PRISM_MD_E_INVALID; }
+
+int prism_md_batch_slot_info(PrismMdBatch *b, int32_t slot, PrismMdSlotCapsV1 *caps, int32_t *occupied) {
+    if (!b || slot < 0 || slot >= b->eng.n_systems()) return PRISM_MD_E_INVALID;
+    if (caps) *caps = b->eng.slot_caps(slot);
+    if (occupied) *occupied = b->eng.occupied(slot) ? 1 : 0;
+    return PRISM_MD_OK;
+}
+
+int prism_md_batch_device_bytes(PrismMdBatch *b, uint64_t *bytes) {
+    if (!b || !bytes) return PRISM_MD_E_INVALID;
+    *bytes = b->eng.device_bytes();
+    return PRISM_MD_OK;
+}
 
 int prism_md_time_nonbonded(PrismMdHandle *h, int iters, double *mean_ms) {
     return h && mean_ms && iters > 0 ? h->eng.time_nonbonded(iters, mean_ms) : PRISM_MD_E_INVALID;

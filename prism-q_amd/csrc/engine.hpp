@@ -30,6 +30,15 @@ class Engine {
     // Capacity in atoms / groups / terms is fixed at construction; systems may
     // be (de)activated later through set_active.
     int init(const std::vector<SystemSpec> &systems, void *stream);
+    // Empty slots of the given capacities (serving pool); systems enter with
+    // admit and leave with retire, without reallocation and without
+    // invalidating a captured graph.
+    int init_pool(const std::vector<PrismMdSlotCapsV1> &caps, void *stream);
+    int admit(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p);
+    int retire(int s);
+    bool occupied(int s) const { return desc_[s].active != 0; }
+    const PrismMdSlotCapsV1 &slot_caps(int s) const { return slots_[s].caps; }
+    size_t device_bytes() const { return bytes_; }
 
     int n_systems() const { return (int)sys_.size(); }
     int n_atoms(int s) const { return sys_[s].n_atoms; }
@@ -64,6 +73,10 @@ class Engine {
     int d2h(void *dst, const void *src, size_t bytes);  // stream-ordered copy + synchronise
     int refresh_all();            // rebuild lists + forces at current positions
     int refresh(int only);        // the same for one system (-1: all)
+    int layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream);
+    int lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p);
+    int upload_slot(int s);
+    template <class T> int upload_range(T *d, const std::vector<T> &h, size_t begin, size_t count);
     void launch_step(const float *noise_n, const float *noise_np1);
     void full_force();
     void harvest_timing();
@@ -79,6 +92,39 @@ class Engine {
     };
     std::vector<HostSys> sys_;
     std::vector<SysDesc> desc_;
+    // index ranges of one slot in every arena array
+    struct Slot {
+        int atom_base, atom_cap, wave_base, wave_cap;
+        int bond_base, bond_cap, angle_base, angle_cap, dih_base, dih_cap, exc_base, exc_cap;
+        int skip_base, skip_cap, ex_base, ex_cap, contrib_base, contrib_cap;
+        PrismMdSlotCapsV1 caps;
+    };
+    std::vector<Slot> slots_;
+    // host mirror of the arena arrays written by lower()
+    struct HostArena {
+        std::vector<float4> x, v;
+        std::vector<float> mass, q, sig, eps;
+        std::vector<int> atom_sys, block_sys, wave_sys;
+        std::vector<int> skip_off, skip_idx, ex_off, ex_idx, inc_off;
+        std::vector<Group> groups;
+        std::vector<GroupParam> gp;
+        std::vector<Bond> bonds;
+        std::vector<Angle> angles;
+        std::vector<Dihedral> dih;
+        std::vector<Exception14> exc;
+        std::vector<int> dst_b, dst_a, dst_d, dst_e;
+    } h_;
+    float *d_mass_ = nullptr, *d_q_ = nullptr, *d_sig_ = nullptr, *d_eps_ = nullptr;
+    int *d_atom_sys_ = nullptr, *d_block_sys_ = nullptr, *d_wave_sys_ = nullptr;
+    int *d_skip_off_ = nullptr, *d_skip_idx_ = nullptr, *d_ex_off_ = nullptr, *d_ex_idx_ = nullptr, *d_inc_off_ = nullptr;
+    int *d_dst_b_ = nullptr, *d_dst_a_ = nullptr, *d_dst_d_ = nullptr, *d_dst_e_ = nullptr;
+    Group *d_groups_ = nullptr;
+    GroupParam *d_gp_ = nullptr;
+    Bond *d_bonds_ = nullptr;
+    Angle *d_angles_ = nullptr;
+    Dihedral *d_dih_ = nullptr;
+    Exception14 *d_exc_ = nullptr;
+    size_t bytes_ = 0;
     DevPtrs P_{};
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;

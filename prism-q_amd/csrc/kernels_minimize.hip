@@ -17,7 +17,7 @@ constexpr int MT = 256;
 __global__ __launch_bounds__(MT) void k_fire_partials(DevPtrs P, const float4 *vmin, double *part) {
     const int a = blockIdx.x * MT + threadIdx.x;
     double q[4] = {0.0, 0.0, 0.0, 0.0};
-    if (a < P.n_atoms_arena && P.atom_sys[a] >= 0) {
+    if (a < P.n_atoms_arena && P.atom_sys[a] >= 0 && P.sys[P.atom_sys[a]].active) {
         const long long *fa = P.f + (size_t)a * 4;
         const double s = 1.0 / (double)(1ll << FORCE_FIXED_BITS);
         const double fx = fa[0] * s, fy = fa[1] * s, fz = fa[2] * s;
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(MT) void k_snap_hydrogens(DevPtrs P) {
     const int g = blockIdx.x * MT + threadIdx.x;
     if (g >= P.n_waves_arena * 64) return;
     const Group G = P.groups[g];
-    if (G.atom[0] < 0) return;
+    if (G.atom[0] < 0 || !P.sys[P.atom_sys[G.atom[0]]].active) return;
     const GroupParam gp = P.gparams[g];
     const float4 c = P.x[G.atom[0]];
     for (int h = 0; h < gp.n_h; ++h) {

@@ -71,6 +71,11 @@ class ParamsV1(C.Structure):
     ]
 
 
+class SlotCapsV1(C.Structure):
+    _fields_ = [("atoms", C.c_int32), ("bonds", C.c_int32), ("angles", C.c_int32), ("dihedrals", C.c_int32),
+                ("exceptions", C.c_int32), ("exclusions", C.c_int32), ("reserved", C.c_int32 * 2)]
+
+
 class StatsV1(C.Structure):
     _fields_ = [("steps_done", C.c_uint64), ("pair_rebuilds", C.c_uint64), ("force_evaluations", C.c_uint64),
                 ("n_tiles", C.c_int32), ("n_blocks", C.c_int32), ("status", C.c_int32),
@@ -145,6 +150,12 @@ def md_lib():
             "prism_md_batch_energies": ([H, C.c_int32, C.POINTER(C.c_double)], C.c_int),
             "prism_md_batch_get_stats": ([H, C.c_int32, C.POINTER(StatsV1)], C.c_int),
             "prism_md_batch_last_error": ([H], C.c_char_p),
+            "prism_md_slot_caps_for": ([C.POINTER(TopologyV1), C.POINTER(SlotCapsV1)], C.c_int),
+            "prism_md_pool_create": ([C.POINTER(SlotCapsV1), C.c_int32, C.c_void_p, C.POINTER(H)], C.c_int),
+            "prism_md_batch_admit": ([H, C.c_int32, C.POINTER(TopologyV1), C.POINTER(ParamsV1)], C.c_int),
+            "prism_md_batch_retire": ([H, C.c_int32], C.c_int),
+            "prism_md_batch_slot_info": ([H, C.c_int32, C.POINTER(SlotCapsV1), C.POINTER(C.c_int32)], C.c_int),
+            "prism_md_batch_device_bytes": ([H, C.POINTER(C.c_uint64)], C.c_int),
             "prism_md_minimize": ([H, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         }
         for name, (args, res) in sig.items():
@@ -409,8 +420,11 @@ class Ensemble:
     system keeps its own parameters, noise stream, rebuilds and faults; its
     trajectory is bitwise the one it follows alone."""
 
-    def __init__(self, topologies, params=None, hip_stream: int | None = None):
+    def __init__(self, topologies, params=None, hip_stream: int | None = None, _handle=None):
         self.lib = md_lib()
+        if _handle is not None:  # a pool: slots filled by admit()
+            self.h, self.topologies, self.params, self._structs = _handle, list(topologies), list(params), []
+            return
         self.topologies = list(topologies)
         n = len(self.topologies)
         if params is None:
@@ -498,3 +512,59 @@ class Ensemble:
         st = StatsV1()
         self._check(self.lib.prism_md_batch_get_stats(self.h, s, C.byref(st)), "batch_stats")
         return {f: getattr(st, f) for f, _ in StatsV1._fields_ if f != "reserved"}
+
+
+def slot_caps_for(topology: Topology, atoms: int | None = None, scale: float = 1.0) -> SlotCapsV1:
+    """Exact capacities of `topology`; with `atoms`, the same per-atom
+    proportions scaled to hold `atoms - 1` atoms (times `scale` headroom)."""
+    c = SlotCapsV1()
+    rc = md_lib().prism_md_slot_caps_for(C.byref(topology.struct()), C.byref(c))
+    if rc != OK:
+        raise PrismMdError(rc, "slot_caps_for")
+    if atoms is not None:
+        f = scale * (atoms - 1) / max(1, topology.n_atoms)
+        for k in ("bonds", "angles", "dihedrals", "exceptions", "exclusions"):
+            setattr(c, k, int(np.ceil(getattr(c, k) * f)))
+        c.atoms = int(atoms)
+    return c
+
+
+class ServingPool(Ensemble):
+    """Slots of fixed capacity, filled and emptied while the others keep
+    running (prism_md_pool_create / batch_admit / batch_retire)."""
+
+    def __init__(self, caps, hip_stream: int | None = None):
+        lib = md_lib()
+        arr = (SlotCapsV1 * len(caps))(*caps)
+        h = C.c_void_p()
+        rc = lib.prism_md_pool_create(arr, len(caps), C.c_void_p(hip_stream) if hip_stream else None, C.byref(h))
+        if rc != OK:
+            raise PrismMdError(rc, "prism_md_pool_create: " + lib.prism_md_batch_last_error(None).decode())
+        super().__init__([None] * len(caps), [None] * len(caps), _handle=h)
+        self.caps = list(caps)
+
+    def admit(self, slot: int, topology: Topology, params: ParamsV1 | None = None):
+        p = params if params is not None else default_params()
+        self._check(self.lib.prism_md_batch_admit(self.h, slot, C.byref(topology.struct()), C.byref(p)), "admit")
+        self.topologies[slot] = topology
+        self.params[slot] = p
+
+    def retire(self, slot: int):
+        self._check(self.lib.prism_md_batch_retire(self.h, slot), "retire")
+
+    def occupied(self, slot: int) -> bool:
+        occ = C.c_int32()
+        self._check(self.lib.prism_md_batch_slot_info(self.h, slot, None, C.byref(occ)), "slot_info")
+        return bool(occ.value)
+
+    def fits(self, slot: int, topology: Topology) -> bool:
+        need = slot_caps_for(topology)
+        c = self.caps[slot]
+        return (topology.n_atoms < c.atoms and need.bonds <= c.bonds and need.angles <= c.angles
+                and need.dihedrals <= c.dihedrals and need.exceptions <= c.exceptions
+                and need.exclusions <= c.exclusions)
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        self._check(self.lib.prism_md_batch_device_bytes(self.h, C.byref(b)), "device_bytes")
+        return int(b.value)

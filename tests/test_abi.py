@@ -58,6 +58,7 @@ def _c_layout(struct_fields):
 
 def test_struct_layouts_match_ctypes(pm):
     structs = {"PrismMdTopologyV1": pm.TopologyV1, "PrismMdParamsV1": pm.ParamsV1, "PrismMdStatsV1": pm.StatsV1,
+               "PrismMdSlotCapsV1": pm.SlotCapsV1,
                "PrismGenParamsV1": pm.GenParamsV1, "PrismGenStatsV1": pm.GenStatsV1}
     fields = {s: [f for f, _ in t._fields_] for s, t in structs.items()}
     c = _c_layout(fields)
