@@ -68,15 +68,16 @@ __device__ inline void langevin_normals(uint32_t seed_lo, uint32_t stream, uint3
                                         uint32_t purpose, float out[3]) {
     uint32_t c[4] = {atom, (uint32_t)step, (uint32_t)(((step >> 32) & 0xFFFFull) << 16) | ((purpose & 0xFFu) << 8), 0u};
     philox4x32_10(c, seed_lo, stream);
-    const float s = 2.3283064365386963e-10f;
-    const float a = sqrtf(-2.0f * logf(fmaf((float)c[0], s, s)));
-    const float b = sqrtf(-2.0f * logf(fmaf((float)c[2], s, s)));
-    float sa, ca, sb, cb;
-    sincosf(6.283185307179586f * ((float)c[1] * s), &sa, &ca);
-    sincosf(6.283185307179586f * ((float)c[3] * s), &sb, &cb);
-    out[0] = a * ca;
-    out[1] = a * sa;
-    out[2] = b * cb;
+    // Box-Muller on the hardware transcendentals: v_log_f32 is log2, and
+    // v_sin_f32 / v_cos_f32 take the angle in revolutions, which is exactly
+    // the uniform u2 of the transform (angle 2 pi u2).
+    const float s = 2.3283064365386963e-10f;  // 2^-32
+    const float a = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(fmaf((float)c[0], s, s)));  // -2 ln u1
+    const float b = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(fmaf((float)c[2], s, s)));
+    const float t1 = (float)c[1] * s, t3 = (float)c[3] * s;
+    out[0] = a * __builtin_amdgcn_cosf(t1);
+    out[1] = a * __builtin_amdgcn_sinf(t1);
+    out[2] = b * __builtin_amdgcn_cosf(t3);
 }
 
 // Director schedule (protocol_director.cu:58-123): temperature table and the

@@ -33,98 +33,181 @@ struct GroupState {
 #define FOR_ATOMS(k) _Pragma("unroll") for (int k = 0; k < 4; ++k) if (k < g.n)
 #define FOR_H(h) _Pragma("unroll") for (int h = 1; h < 4; ++h) if (h < g.n)
 
-// rattle_cluster_velocities (nhs_amber_fused.cu:2175-2360), FP64 iteration.
-// Returns the sweeps used, or -1 when the iteration fails.
-__device__ int rattle(GroupState &g, const GroupParam &gp, double tol, int max_iter) {
-    if (g.n < 2) return 0;
-    double x[4][3], v[4][3];
-    FOR_ATOMS(k) {
-        for (int c = 0; c < 3; ++c) {
-            x[k][c] = g.x[k][c];
-            v[k][c] = g.v[k][c];
-        }
+// Solve the k x k (k <= 3) system M lam = b of one cluster, M symmetric
+// positive definite, by cofactors in FP64.  Returns false when M is singular.
+__device__ inline bool solve3(int k, const double M[3][3], const double b[3], double lam[3]) {
+    if (k == 1) {
+        if (!(M[0][0] > 0.0)) return false;
+        lam[0] = b[0] / M[0][0];
+        return true;
     }
-    const double imc = gp.inv_mass_c, imh = gp.inv_mass_h;
-    int used = 0;
-    bool converged = false;
-    while (!converged && used < max_iter) {
-        ++used;
-        double max_err = 0.0;
-        bool bad = false;
-        FOR_H(h) {
-            const double rx = x[h][0] - x[0][0], ry = x[h][1] - x[0][1], rz = x[h][2] - x[0][2];
-            const double r2 = rx * rx + ry * ry + rz * rz;
-            const double ux = v[h][0] - v[0][0], uy = v[h][1] - v[0][1], uz = v[h][2] - v[0][2];
-            const double res = rx * ux + ry * uy + rz * uz;
-            const double rel = fabs(res) / (sqrt(r2) * fmax(sqrt(ux * ux + uy * uy + uz * uz), 1.0));
-            bad |= !(r2 >= 1e-12) || !isfinite(rel);
-            max_err = fmax(max_err, rel);
-            if (rel > tol) {
-                const double lam = res / ((imc + imh) * r2);
-                v[0][0] += imc * lam * rx; v[0][1] += imc * lam * ry; v[0][2] += imc * lam * rz;
-                v[h][0] -= imh * lam * rx; v[h][1] -= imh * lam * ry; v[h][2] -= imh * lam * rz;
-            }
-        }
-        if (bad) return -1;
-        converged = max_err <= tol;
+    if (k == 2) {
+        const double det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+        if (!(fabs(det) > 1e-300)) return false;
+        const double id = 1.0 / det;
+        lam[0] = (b[0] * M[1][1] - M[0][1] * b[1]) * id;
+        lam[1] = (M[0][0] * b[1] - b[0] * M[1][0]) * id;
+        return true;
     }
-    FOR_ATOMS(k) {
-        for (int c = 0; c < 3; ++c) g.v[k][c] = (float)v[k][c];
-    }
-    return converged ? used : -1;
+    const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+    const double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+    const double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+    const double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+    if (!(fabs(det) > 1e-300)) return false;
+    const double id = 1.0 / det;
+    const double c10 = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+    const double c11 = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+    const double c12 = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+    const double c20 = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+    const double c21 = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+    const double c22 = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    lam[0] = (c00 * b[0] + c10 * b[1] + c20 * b[2]) * id;
+    lam[1] = (c01 * b[0] + c11 * b[1] + c21 * b[2]) * id;
+    lam[2] = (c02 * b[0] + c12 * b[1] + c22 * b[2]) * id;
+    return true;
 }
 
-// shake_cluster_positions (nhs_amber_fused.cu:1889-2173), FP64 iteration:
-// corrections along the step-start bond, velocity impulse correction/dt.
-// Returns the sweeps used, or -1 when the iteration fails.
+// RATTLE of one cluster (rattle_cluster_velocities, nhs_amber_fused.cu:2175-2360).
+// The reference iterates Gauss-Seidel over the X-H bonds with the update
+// v_c += m_c^-1 lam r, v_h -= m_h^-1 lam r until r . (v_h - v_c) vanishes;
+// those conditions are linear in the multipliers, so here the k x k system
+//   sum_j [m_c^-1 (r_h . r_j) + m_h^-1 delta_hj (r_h . r_h)] lam_j = r_h . (v_h - v_c)
+// is solved exactly in FP64: the fixed point of that iteration in one step.
+// Returns 1, or -1 when the cluster is degenerate or the result misses tol.
+__device__ int rattle(GroupState &g, const GroupParam &gp, double tol, int max_iter) {
+    (void)max_iter;
+    if (g.n < 2) return 0;
+    const int k = g.n - 1;
+    const double imc = gp.inv_mass_c, imh = gp.inv_mass_h;
+    double r[3][3] = {}, u[3][3] = {}, M[3][3] = {}, b[3] = {}, lam[3] = {};
+    FOR_H(h) {
+        for (int c = 0; c < 3; ++c) {
+            r[h - 1][c] = (double)g.x[h][c] - (double)g.x[0][c];
+            u[h - 1][c] = (double)g.v[h][c] - (double)g.v[0][c];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (i < k) {
+            b[i] = r[i][0] * u[i][0] + r[i][1] * u[i][1] + r[i][2] * u[i][2];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (j < k) {
+                    const double rr = r[i][0] * r[j][0] + r[i][1] * r[j][1] + r[i][2] * r[j][2];
+                    M[i][j] = imc * rr + (i == j ? imh * rr : 0.0);
+                }
+        }
+    if (!solve3(k, M, b, lam)) return -1;
+    double dv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        if (j < k)
+            for (int c = 0; c < 3; ++c) dv[c] += lam[j] * r[j][c];
+    double vc[3];
+    for (int c = 0; c < 3; ++c) vc[c] = (double)g.v[0][c] + imc * dv[c];
+    double worst = 0.0;
+    bool bad = false;
+    FOR_H(h) {
+        double vh[3];
+        for (int c = 0; c < 3; ++c) vh[c] = (double)g.v[h][c] - imh * lam[h - 1] * r[h - 1][c];
+        const double ux = vh[0] - vc[0], uy = vh[1] - vc[1], uz = vh[2] - vc[2];
+        const double r2 = r[h - 1][0] * r[h - 1][0] + r[h - 1][1] * r[h - 1][1] + r[h - 1][2] * r[h - 1][2];
+        const double res = r[h - 1][0] * ux + r[h - 1][1] * uy + r[h - 1][2] * uz;
+        const double rel = fabs(res) / (sqrt(r2) * fmax(sqrt(ux * ux + uy * uy + uz * uz), 1.0));
+        bad |= !(r2 >= 1e-12) || !isfinite(rel);
+        worst = fmax(worst, rel);
+        for (int c = 0; c < 3; ++c) g.v[h][c] = (float)vh[c];
+    }
+    for (int c = 0; c < 3; ++c) g.v[0][c] = (float)vc[c];
+    return (bad || !(worst <= fmax(tol, 1e-12))) ? -1 : 1;
+}
+
+// SHAKE of one cluster (shake_cluster_positions, nhs_amber_fused.cu:1889-2173).
+// The reference moves the atoms along the step-start bond vectors o_h,
+//   x_c += m_c^-1 lam_h o_h,  x_h -= m_h^-1 lam_h o_h,  v += dx / dt,
+// sweeping Gauss-Seidel until |r_h|^2 = d_h^2.  Here the same k equations in
+// the k multipliers are solved by Newton's method in FP64 (Jacobian
+// dg_h/dlam_j = -2 (m_h^-1 delta_hj r_h . o_h + m_c^-1 r_h . o_j)), which
+// reaches that fixed point in a few iterations.  Returns the iterations used,
+// or -1 on failure.
 __device__ int shake(GroupState &g, const float xs[4][3], const GroupParam &gp, float dt, double tol, int max_iter) {
     if (g.n < 2) return 0;
-    double x[4][3], v[4][3], o[4][3];
-    FOR_ATOMS(k) {
+    const int k = g.n - 1;
+    const double imc = gp.inv_mass_c, imh = gp.inv_mass_h;
+    double o[3][3] = {}, r0[3][3] = {}, lam[3] = {0.0, 0.0, 0.0}, d2[3] = {};
+    FOR_H(h) {
         for (int c = 0; c < 3; ++c) {
-            x[k][c] = g.x[k][c];
-            v[k][c] = g.v[k][c];
+            o[h - 1][c] = (double)xs[h][c] - (double)xs[0][c];
+            r0[h - 1][c] = (double)g.x[h][c] - (double)g.x[0][c];
         }
+        d2[h - 1] = (double)gp.d[h - 1] * (double)gp.d[h - 1];
+    }
+    int used = 0;
+    bool converged = false, bad = false;
+    while (!converged && !bad && used < max_iter) {
+        ++used;
+        // current bonds r_h(lam) = r0_h - m_h^-1 lam_h o_h - m_c^-1 sum_j lam_j o_j
+        double sc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (j < k)
+                for (int c = 0; c < 3; ++c) sc[c] += lam[j] * o[j][c];
+        double r[3][3] = {}, gres[3] = {}, M[3][3] = {}, dl[3] = {};
+        double worst = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if (i < k) {
+                for (int c = 0; c < 3; ++c) r[i][c] = r0[i][c] - imh * lam[i] * o[i][c] - imc * sc[c];
+                const double r2 = r[i][0] * r[i][0] + r[i][1] * r[i][1] + r[i][2] * r[i][2];
+                gres[i] = r2 - d2[i];
+                worst = fmax(worst, fabs(gres[i]) / d2[i]);
+                bad |= !isfinite(r2) || !(d2[i] > 0.0);
+            }
+        if (bad) break;
+        if (worst <= tol) {
+            converged = true;
+            break;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if (i < k) {
+                const double ro = r[i][0] * o[i][0] + r[i][1] * o[i][1] + r[i][2] * o[i][2];
+                // the reference's guard against a bond that rotated ~90 degrees
+                const double scale = sqrt((r[i][0] * r[i][0] + r[i][1] * r[i][1] + r[i][2] * r[i][2]) *
+                                          (o[i][0] * o[i][0] + o[i][1] * o[i][1] + o[i][2] * o[i][2]));
+                bad |= !(scale > 0.0) || fabs(ro) / scale <= 1e-6;
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (j < k) {
+                        const double rj = r[i][0] * o[j][0] + r[i][1] * o[j][1] + r[i][2] * o[j][2];
+                        M[i][j] = 2.0 * (imc * rj + (i == j ? imh * ro : 0.0));  // -dg_i/dlam_j
+                    }
+            }
+        if (bad || !solve3(k, M, gres, dl)) {
+            bad = true;
+            break;
+        }
+        for (int i = 0; i < 3; ++i) lam[i] += dl[i];  // g + J dl = 0 with J = -M
+    }
+    if (!converged) return -1;
+    const double idt = 1.0 / (double)dt;
+    double sc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        if (j < k)
+            for (int c = 0; c < 3; ++c) sc[c] += lam[j] * o[j][c];
+    for (int c = 0; c < 3; ++c) {
+        g.x[0][c] = (float)((double)g.x[0][c] + imc * sc[c]);
+        g.v[0][c] = (float)((double)g.v[0][c] + imc * sc[c] * idt);
     }
     FOR_H(h) {
-        for (int c = 0; c < 3; ++c) o[h][c] = (double)xs[h][c] - (double)xs[0][c];
-    }
-    const double imc = gp.inv_mass_c, imh = gp.inv_mass_h, idt = 1.0 / (double)dt;
-    int used = 0;
-    bool converged = false;
-    while (!converged && used < max_iter) {
-        ++used;
-        double max_err = 0.0;
-        bool bad = false;
-        FOR_H(h) {
-            const double rx = x[h][0] - x[0][0], ry = x[h][1] - x[0][1], rz = x[h][2] - x[0][2];
-            const double d = gp.d[h - 1];
-            const double r2 = rx * rx + ry * ry + rz * rz, d2 = d * d, diff = r2 - d2;
-            const double rel = fabs(diff) / d2;
-            bad |= !isfinite(r2) || !(d2 > 0.0);
-            max_err = fmax(max_err, rel);
-            if (rel > tol) {
-                const double ox = o[h][0], oy = o[h][1], oz = o[h][2];
-                const double rr0 = rx * ox + ry * oy + rz * oz;
-                const double scale = sqrt(r2 * (ox * ox + oy * oy + oz * oz));
-                bad |= !isfinite(rr0) || !(scale > 0.0) || fabs(rr0) / scale <= 1e-6;
-                const double lam = diff / (2.0 * (imc + imh) * rr0);
-                x[0][0] += imc * lam * ox; x[0][1] += imc * lam * oy; x[0][2] += imc * lam * oz;
-                x[h][0] -= imh * lam * ox; x[h][1] -= imh * lam * oy; x[h][2] -= imh * lam * oz;
-                v[0][0] += imc * lam * ox * idt; v[0][1] += imc * lam * oy * idt; v[0][2] += imc * lam * oz * idt;
-                v[h][0] -= imh * lam * ox * idt; v[h][1] -= imh * lam * oy * idt; v[h][2] -= imh * lam * oz * idt;
-            }
-        }
-        if (bad) return -1;
-        converged = max_err <= tol;
-    }
-    FOR_ATOMS(k) {
         for (int c = 0; c < 3; ++c) {
-            g.x[k][c] = (float)x[k][c];
-            g.v[k][c] = (float)v[k][c];
+            const double dxh = imh * lam[h - 1] * o[h - 1][c];
+            g.x[h][c] = (float)((double)g.x[h][c] - dxh);
+            g.v[h][c] = (float)((double)g.v[h][c] - dxh * idt);
         }
     }
-    return converged ? used : -1;
+    return used;
 }
 
 // langevin_thermostat with dt/2 (nhs_amber_fused.cu:1713-1744, :3656-3666).
@@ -240,7 +323,6 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
             if (!(finite3(g.x[k]) && finite3(g.v[k]))) record_fault(S, FAULT_NONFINITE, a, step);
             P.x[a] = make_float4(g.x[k][0], g.x[k][1], g.x[k][2], 0.0f);
             P.v[a] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
-            P.xs[a] = make_float4(xs[k][0], xs[k][1], xs[k][2], 0.0f);
             float *q = (float *)&P.xq[P.slot_of[a]];
             q[0] = g.x[k][0]; q[1] = g.x[k][1]; q[2] = g.x[k][2];
             const float4 b = P.x_build[a];
