@@ -28,22 +28,24 @@ __device__ inline bool in_shell(float r2, unsigned lo_bits, unsigned span) {
     return (unsigned)(__float_as_uint(r2) - lo_bits) < span;
 }
 
+// The survivors of a tile sit in LDS padded to a multiple of 4 with dummy
+// j-atoms far outside the cutoff, so every lane's j index is valid.  Kept
+// pairs have r2 >= 0.01 (never denormal), rejected ones are zeroed by the
+// select, so the raw v_rsq_f32 is used without denormal scaling.
 template <bool ENERGY, bool MASKED>
-__device__ inline void tile_pairs(const float4 xi, const float2 li, int n_act, int js, const float4 *jpos,
+__device__ inline void tile_pairs(const float4 xi, const float2 li, int n_iter, int js, const float4 *jpos,
                                   const float4 *jpar, unsigned long long excl, unsigned lo_bits, unsigned span,
                                   float inv_rc, float inv_rc2, float &fx, float &fy, float &fz, float &e) {
-    const int n_iter = (n_act + 3) >> 2;
 #pragma unroll 2
     for (int k = 0; k < n_iter; ++k) {
         const int jj = 4 * k + js;
-        const bool valid = jj < n_act;
-        const float4 q = jpos[valid ? jj : 0];
-        const float4 pr = jpar[valid ? jj : 0];
+        const float4 q = jpos[jj];
+        const float4 pr = jpar[jj];
         const float dx = xi.x - q.x, dy = xi.y - q.y, dz = xi.z - q.z;
         const float r2 = pair_r2(dx, dy, dz);
-        bool keep = valid && in_shell(r2, lo_bits, span);
+        bool keep = in_shell(r2, lo_bits, span);
         if (MASKED) keep = keep && !((excl >> __float_as_int(pr.z)) & 1ull);
-        const float inv_r = rsqrtf(r2);
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
         const float inv_r2 = inv_r * inv_r;
         const float sig = li.x + pr.x;
         const float s2 = sig * sig * inv_r2;
@@ -146,12 +148,18 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                     jpos[pos] = xj;
                     jpar[pos] = make_float4(lj.x, lj.y, __int_as_float(lane), 0.0f);
                 }
+                const int n_pad = (n_act + 3) & ~3;
+                if (lane >= n_act && lane < n_pad) {  // dummies: r2 ~ 1e14, outside every cutoff
+                    jpos[lane] = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f);
+                    jpar[lane] = make_float4(0.0f, 0.0f, __int_as_float(0), 0.0f);
+                }
                 wave_lds_fence();
+                const int n_iter = n_pad >> 2;
                 if (ms >= 0) {
                     const unsigned long long excl = P.masks[((size_t)b * MAX_MASKS + ms) * 64 + q * SUB + il];
-                    tile_pairs<ENERGY, true>(xi, li, n_act, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
+                    tile_pairs<ENERGY, true>(xi, li, n_iter, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
                 } else {
-                    tile_pairs<ENERGY, false>(xi, li, n_act, js, jpos, jpar, 0ull, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
+                    tile_pairs<ENERGY, false>(xi, li, n_iter, js, jpos, jpar, 0ull, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
                 }
                 wave_lds_fence();
             }
