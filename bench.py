@@ -108,23 +108,45 @@ def ensemble_leg(pm, n_systems, steps, graph_steps, seed=0x50524953 + 3):
     ens = pm.Ensemble(tops, params)
     ens.minimize(1000, 1.0)
     starts = [ens.get_positions(k) for k in range(n_systems)]
-    ens.step(2 * graph_steps, graph_steps)  # warm-up, captures the graph
+    bad = []
+
+    def advance(n):
+        """n steps of every running system; a system that latches a fault
+        halts there (the others run on) and is switched off here."""
+        try:
+            ens.step(n, graph_steps)
+        except pm.PrismMdError:
+            for k in range(n_systems):
+                if k not in bad and ens.stats(k)["status"] != 0:
+                    bad.append(k)
+                    ens.set_active(k, False)
+            if not bad:
+                raise
+
+    advance(2 * graph_steps)  # warm-up, captures the graph
     # per-graph-launch latency: each replay runs graph_steps steps of every system
     lat = []
+    atom_steps = 0
     t0 = time.perf_counter()
     for _ in range(steps // graph_steps):
+        running = total - int(sum(sizes[k] for k in bad))
         t1 = time.perf_counter()
-        ens.step(graph_steps, graph_steps)
+        advance(graph_steps)
         lat.append((time.perf_counter() - t1) / graph_steps)
+        atom_steps += running * graph_steps
     wall_b = time.perf_counter() - t0
     done_b = (steps // graph_steps) * graph_steps
-    bad = [k for k in range(n_systems) if ens.stats(k)["status"] != 0]
     ens.close()
-    batched = total * done_b / wall_b
+    batched = atom_steps / wall_b
     # plain path: the same systems one at a time, same starting positions
+    # (systems that faulted in the batch fault at the same step alone: skipped)
     wall_p = 0.0
     plain_lat = []
+    plain_atoms = 0
     for k in range(n_systems):
+        if k in bad:
+            continue
+        plain_atoms += int(sizes[k])
         with pm.FusedEngine(tops[k], params[k]) as eng:
             eng.set_state(x=starts[k])
             eng.step(2 * graph_steps)
@@ -133,7 +155,7 @@ def ensemble_leg(pm, n_systems, steps, graph_steps, seed=0x50524953 + 3):
             dt = time.perf_counter() - t1
             wall_p += dt
             plain_lat.append(dt / done_b)
-    plain = total * done_b / wall_p
+    plain = plain_atoms * done_b / wall_p
     lat = np.array(lat) * 1e3
     return {"n_systems": n_systems, "atoms": total, "steps": done_b, "graph_steps": graph_steps,
             "atom_steps_per_s": round(batched, 1), "plain_atom_steps_per_s": round(plain, 1),

@@ -304,7 +304,12 @@ int prism_md_batch_create(const PrismMdTopologyV1 *const *topologies, const Pris
 int prism_md_batch_destroy(PrismMdBatch *b);
 int prism_md_batch_size(PrismMdBatch *b, int32_t *n_systems, int64_t *n_atoms);
 /* n_steps steps of every active system; graph_steps > 0 captures that many
- * steps once into a hipGraph and replays it (the remainder runs eagerly). */
+ * steps once into a hipGraph and replays it (the remainder runs eagerly).
+ * A system whose step latches a fault (constraint, non-finite state,
+ * capacity) halts at the end of that step and keeps its state there; the
+ * other systems run all n_steps, and the call returns the fault of the first
+ * active faulted system.  Retiring or deactivating the faulted system clears
+ * the error for later calls (its stats keep the fault bits). */
 int prism_md_batch_step(PrismMdBatch *b, uint64_t n_steps, int32_t graph_steps);
 /* Joint FIRE minimisation of all active systems (see prism_md_minimize). */
 int prism_md_batch_minimize(PrismMdBatch *b, int max_iter, double f_tol, double *f_max);

@@ -701,7 +701,9 @@ int Engine::check_status() {
     if (int rc = d2h(st.data(), P_.sst, st.size() * sizeof(SysState))) return rc;
     for (size_t s = 0; s < st.size(); ++s) {
         const int b = st[s].status;
-        if (!b) continue;
+        // a retired or deactivated system keeps its latched fault for
+        // get_stats but no longer fails calls on the systems still running
+        if (!b || !desc_[s].active) continue;
         static const char *kinds[] = {"?", "RATTLE after OU_N", "RATTLE before drift", "SHAKE", "RATTLE before OU_NP1",
                                       "final RATTLE", "non-finite state"};
         const int fk = st[s].fault_kind >= 0 && st[s].fault_kind <= 6 ? st[s].fault_kind : 0;

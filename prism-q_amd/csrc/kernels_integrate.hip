@@ -292,6 +292,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
     const SysDesc &D = P.sys[s];
     if (!D.active) return;
     SysState &S = P.sst[s];
+    if (S.halted) return;
     const unsigned long long step = S.step;
     float T, gamma;
     schedule(D, step + 1, T, gamma);
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
     const int gw = blockIdx.x * IW + (threadIdx.x >> 6);
     __shared__ int last;
     const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
-    if (s >= 0 && P.sys[s].active) {
+    if (s >= 0 && P.sys[s].active && !P.sst[s].halted) {
         const SysDesc &D = P.sys[s];
         SysState &S = P.sst[s];
         const unsigned long long step = S.step;
@@ -393,6 +394,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         const SysDesc &D = P.sys[q];
         if (!D.active) continue;
         SysState &S = P.sst[q];
+        if (S.halted) continue;
         const unsigned long long step = S.step;
         if (D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0) {
             double acc[3] = {0.0, 0.0, 0.0};
@@ -407,6 +409,8 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
             S.n_rebuilds += 1;
         }
         S.step = step + 1;
+        // a latched fault stops this system here; the others keep running
+        if (S.status & ST_HALT) S.halted = 1;
     }
     if (threadIdx.x == 0) {
         P.counters->done_blocks = 0;

@@ -41,6 +41,8 @@ enum : int {
     ST_ROW_OVERFLOW = 4, // more than MAX_ROW tiles in a row
     ST_MASK_OVERFLOW = 8,
     ST_EXCEPTION_COINCIDENT = 16,
+    // any of these halts the system's integration at the end of the step
+    ST_HALT = ST_CONSTRAINT | ST_NONFINITE | ST_ROW_OVERFLOW | ST_MASK_OVERFLOW | ST_EXCEPTION_COINCIDENT,
 };
 
 // Immutable per-system description (host-written).
@@ -88,6 +90,9 @@ struct SysState {
     unsigned long long fault_step;
     int max_shake_iter;             // most SHAKE sweeps any group needed
     int max_rattle_iter;            // most RATTLE sweeps any group needed
+    int halted;                     // set at the end of a step that latched an ST_HALT bit: the
+                                    // integrator leaves the system as it was after that step
+    int pad_state;
 };
 
 // failure kinds recorded with the first fault

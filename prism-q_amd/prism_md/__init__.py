@@ -97,6 +97,18 @@ class GenStatsV1(C.Structure):
                 ("total_charge", C.c_double), ("total_mass", C.c_double)]
 
 
+class TopoLoadOptionsV1(C.Structure):
+    _fields_ = [("harmonic_convention", C.c_int32), ("apply_hmr", C.c_int32), ("hmr_factor", C.c_double),
+                ("reserved", C.c_int32 * 8)]
+
+
+class TopoInfoV1(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("n_residues", C.c_int32), ("n_chains", C.c_int32),
+                ("hmr_in_file", C.c_int32), ("hmr_applied_here", C.c_int32), ("hmr_skipped", C.c_int32),
+                ("harmonic_convention", C.c_int32), ("has_exception_contract", C.c_int32),
+                ("recommended_timestep_fs", C.c_double), ("reserved", C.c_int32 * 8)]
+
+
 _LIBS: dict = {}
 
 
@@ -177,6 +189,17 @@ def gen_lib():
         lib.prism_gen_stats.argtypes = [H, C.POINTER(GenStatsV1)]
         lib.prism_gen_destroy.argtypes = [H]
         lib.prism_gen_maxwell_boltzmann.argtypes = [_f32p, C.c_int32, C.c_double, C.c_uint64, _f32p]
+        lib.prism_topo_default_options.argtypes = [C.POINTER(TopoLoadOptionsV1)]
+        lib.prism_topo_load_json.argtypes = [C.c_char_p, C.POINTER(TopoLoadOptionsV1), C.POINTER(H)]
+        lib.prism_topo_parse_json.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(TopoLoadOptionsV1), C.POINTER(H)]
+        lib.prism_topo_view.argtypes = [H]
+        lib.prism_topo_view.restype = C.POINTER(TopologyV1)
+        lib.prism_topo_info.argtypes = [H, C.POINTER(TopoInfoV1)]
+        lib.prism_topo_destroy.argtypes = [H]
+        lib.prism_topo_last_error.argtypes = []
+        lib.prism_topo_last_error.restype = C.c_char_p
+        lib.prism_topo_write_json.argtypes = [C.POINTER(TopologyV1), C.c_int32, C.c_char_p]
+        lib.prism_topo_sha256_hex.argtypes = [C.c_void_p, C.c_size_t, C.c_char_p]
         lib._typed = True
     return lib
 
@@ -202,6 +225,21 @@ _ARRAYS = {  # field -> (dtype, arity, count field)
     "cluster_inv_mass_central": (np.float32, 1, "n_clusters"), "cluster_inv_mass_h": (np.float32, 1, "n_clusters"),
 }
 _CTYPE = {np.float32: C.c_float, np.int32: C.c_int32, np.uint8: C.c_uint8}
+
+
+def _copy_struct(t: TopologyV1) -> dict:
+    """numpy copies of the arrays a PrismMdTopologyV1 points to."""
+    arrays = {}
+    for name, (dt, ar, cnt) in _ARRAYS.items():
+        n = getattr(t, cnt)
+        p = getattr(t, name)
+        arrays[name] = np.ctypeslib.as_array(p, shape=(n * ar,)).astype(dt).copy() if n and p else np.zeros(0, dt)
+    n = t.n_atoms
+    off = np.ctypeslib.as_array(t.exclusion_offsets, shape=(n + 1,)).astype(np.int32).copy()
+    arrays["exclusion_offsets"] = off
+    arrays["exclusion_atoms"] = (np.ctypeslib.as_array(t.exclusion_atoms, shape=(int(off[-1]),))
+                                 .astype(np.int32).copy() if off[-1] else np.zeros(0, np.int32))
+    return arrays
 
 
 class Topology:
@@ -233,24 +271,49 @@ class Topology:
         if rc != OK:
             raise PrismMdError(rc, "generator failed")
         try:
-            t = lib.prism_gen_topology(h).contents
             st = GenStatsV1()
             lib.prism_gen_stats(h, C.byref(st))
-            arrays = {}
-            for name, (dt, ar, cnt) in _ARRAYS.items():
-                n = getattr(t, cnt)
-                p = getattr(t, name)
-                arrays[name] = np.ctypeslib.as_array(p, shape=(n * ar,)).astype(dt).copy() if n and p else np.zeros(0, dt)
-            n = t.n_atoms
-            off = np.ctypeslib.as_array(t.exclusion_offsets, shape=(n + 1,)).astype(np.int32).copy()
-            arrays["exclusion_offsets"] = off
-            arrays["exclusion_atoms"] = (np.ctypeslib.as_array(t.exclusion_atoms, shape=(int(off[-1]),))
-                                         .astype(np.int32).copy() if off[-1] else np.zeros(0, np.int32))
-            top = cls(arrays)
+            top = cls(_copy_struct(lib.prism_gen_topology(h).contents))
             top.gen_stats = {f: getattr(st, f) for f, _ in GenStatsV1._fields_}
             return top
         finally:
             lib.prism_gen_destroy(h)
+
+    @classmethod
+    def from_json(cls, path: str, convention: str | None = None, apply_hmr: bool = True,
+                  hmr_factor: float = 3.0) -> "Topology":
+        """Read a PrismPrepTopology JSON file (the reference's on-disk topology,
+        crates/prism-nhs/src/input.rs:1027-1182) through prism_topo_load_json:
+        the reference's validation, harmonic lowering and HMR.  `convention`
+        None requires the file's harmonic_parameter_contract; "amber" or
+        "openmm" states the coefficient convention of a file without one."""
+        lib = gen_lib()
+        o = TopoLoadOptionsV1()
+        lib.prism_topo_default_options(C.byref(o))
+        o.harmonic_convention = {None: 0, "amber": 1, "openmm": 2}[convention]
+        o.apply_hmr = int(bool(apply_hmr))
+        o.hmr_factor = float(hmr_factor)
+        h = C.c_void_p()
+        rc = lib.prism_topo_load_json(os.fsencode(path), C.byref(o), C.byref(h))
+        if rc != OK:
+            raise PrismMdError(rc, lib.prism_topo_last_error().decode())
+        try:
+            info = TopoInfoV1()
+            lib.prism_topo_info(h, C.byref(info))
+            top = cls(_copy_struct(lib.prism_topo_view(h).contents))
+            top.json_info = {f: getattr(info, f) for f, _ in TopoInfoV1._fields_ if f != "reserved"}
+            return top
+        finally:
+            lib.prism_topo_destroy(h)
+
+    def to_json(self, path: str, hmr_applied: bool | None = True):
+        """Write this topology as a PrismPrepTopology JSON document
+        (prism_topo_write_json); hmr_applied None leaves the key out."""
+        flag = -1 if hmr_applied is None else int(bool(hmr_applied))
+        lib = gen_lib()
+        rc = lib.prism_topo_write_json(C.byref(self.struct()), flag, os.fsencode(path))
+        if rc != OK:
+            raise PrismMdError(rc, lib.prism_topo_last_error().decode())
 
     def struct(self) -> TopologyV1:
         if self._struct is None:

@@ -37,6 +37,7 @@ class Job:
     final_positions: np.ndarray | None = None
     final_velocities: np.ndarray | None = None
     status: int = 0
+    error: str | None = None
 
     @property
     def n_atoms(self) -> int:
@@ -76,6 +77,7 @@ class Scheduler:
         self.queue: collections.deque[Job] = collections.deque()
         self.running: dict[int, Job] = {}
         self.done: list[Job] = []
+        self.failed: list[Job] = []
         self.step_latency: list[float] = []   # seconds per step, one entry per tick
         self.atom_steps = 0
         self.busy = 0.0
@@ -112,10 +114,16 @@ class Scheduler:
         if not self.running:
             return False
         t0 = self.clock()
-        self.pool.step(self.graph_steps, self.graph_steps)
+        fault = None
+        try:
+            self.pool.step(self.graph_steps, self.graph_steps)
+        except RuntimeError as ex:  # PrismMdError: a latched fault halts that job only
+            fault = ex
         dt = self.clock() - t0
         self.busy += dt
         self.step_latency.append(dt / self.graph_steps)
+        if fault is not None:
+            self._retire_faulted(fault)
         self.atom_steps += self.graph_steps * sum(j.n_atoms for j in self.running.values())
         for slot, job in list(self.running.items()):
             job.done_steps += self.graph_steps
@@ -129,6 +137,30 @@ class Scheduler:
                 del self.running[slot]
         return True
 
+    def _retire_faulted(self, fault: Exception):
+        """Retire every running job whose system latched a fault this tick
+        (it halted at the end of its faulting step; the others ran the whole
+        tick).  A failure that names no running job is re-raised."""
+        faulted = []
+        for slot, job in self.running.items():
+            st = self.pool.stats(slot)
+            if st["status"] != 0:
+                faulted.append((slot, job, st))
+        if not faulted:
+            raise fault
+        for slot, job, st in faulted:
+            done_here = max(0, int(st["steps_done"]) - job.done_steps)
+            self.atom_steps += done_here * job.n_atoms
+            job.done_steps = int(st["steps_done"])
+            job.status = int(st["status"])
+            job.error = str(fault) if len(faulted) == 1 else f"device status {job.status}"
+            job.final_positions = self.pool.get_positions(slot)
+            job.final_velocities = self.pool.get_velocities(slot)
+            self.pool.retire(slot)
+            job.t_done = self.clock()
+            self.failed.append(job)
+            del self.running[slot]
+
     def report(self, wall: float) -> dict:
         lat = np.asarray(self.step_latency) * 1e3
         waits = [j.t_admit - j.t_arrive for j in self.done + list(self.running.values()) if j.t_admit is not None]
@@ -140,5 +172,7 @@ class Scheduler:
             "gpu_busy_fraction": round(self.busy / wall, 4) if wall > 0 else 0.0,
             "jobs_done": len(self.done), "jobs_running": len(self.running), "jobs_queued": len(self.queue),
             "mean_admission_wait_s": round(float(np.mean(waits)), 3) if waits else None,
-            "faulted_jobs": [j.jid for j in self.done if j.status != 0],
+            "faulted_jobs": [j.jid for j in self.failed],
+            "faults": [{"job": j.jid, "atoms": j.n_atoms, "at_step": j.done_steps, "error": j.error}
+                       for j in self.failed],
         }

@@ -14,7 +14,8 @@ import pytest
 
 from conftest import ROOT
 
-HEADERS = {"prism_md.h": "libprism_md.so", "prism_gen.h": "libprism_gen.so"}
+HEADERS = {"prism_md.h": "libprism_md.so", "prism_gen.h": "libprism_gen.so",
+           "prism_topology.h": "libprism_gen.so"}
 INCLUDE = os.path.join(ROOT, "include")
 PKG = os.path.join(ROOT, "prism-q_amd")
 
@@ -23,7 +24,7 @@ def declared_functions(header):
     text = open(os.path.join(INCLUDE, header)).read()
     text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
     text = re.sub(r"#define[^\n]*", " ", text)
-    return sorted(set(re.findall(r"\b(prism_(?:md|gen)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(prism_(?:md|gen|topo)_\w+)\s*\(", text)))
 
 
 @pytest.mark.parametrize("header", sorted(HEADERS))
@@ -42,7 +43,7 @@ def test_every_declared_symbol_is_exported(header):
 def _c_layout(struct_fields):
     """sizeof / offsetof of the C structs, from gcc on the public headers."""
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "prism_md.h"', '#include "prism_gen.h"',
-             "int main(void) {"]
+             '#include "prism_topology.h"', "int main(void) {"]
     for s, fields in struct_fields.items():
         lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
         for f in fields:
@@ -59,7 +60,8 @@ def _c_layout(struct_fields):
 def test_struct_layouts_match_ctypes(pm):
     structs = {"PrismMdTopologyV1": pm.TopologyV1, "PrismMdParamsV1": pm.ParamsV1, "PrismMdStatsV1": pm.StatsV1,
                "PrismMdSlotCapsV1": pm.SlotCapsV1,
-               "PrismGenParamsV1": pm.GenParamsV1, "PrismGenStatsV1": pm.GenStatsV1}
+               "PrismGenParamsV1": pm.GenParamsV1, "PrismGenStatsV1": pm.GenStatsV1,
+               "PrismTopoLoadOptionsV1": pm.TopoLoadOptionsV1, "PrismTopoInfoV1": pm.TopoInfoV1}
     fields = {s: [f for f, _ in t._fields_] for s, t in structs.items()}
     c = _c_layout(fields)
     for s, t in structs.items():

@@ -83,3 +83,28 @@ def test_scheduler_end_to_end(pm, jobs, pool):
     for job in sch.done:
         x1, v1, _ = alone(pm, job.topology, job.params, job.done_steps)
         assert np.array_equal(job.final_positions, x1), job.jid
+
+
+def test_faulted_job_halts_alone(pm, jobs, pool):
+    """A job whose SHAKE cannot converge (one Newton sweep against a 1e-14
+    tolerance) halts at the end of its first step and is retired by the
+    scheduler; the job beside it runs to its length bitwise as it runs alone."""
+    from prism_md.serving import Job, Scheduler
+    top_bad = pm.Topology.generate(1500, seed=34)
+    p_bad = pm.default_params(constraint_max_iter=1, shake_tol=1e-14, seed=77, stream_id=9)
+    (tb, pb) = jobs[0]
+    sch = Scheduler(pool, graph_steps=10)
+    sch.submit(Job(0, top_bad, p_bad, 50))
+    sch.submit(Job(1, tb, pb, 60))
+    while sch.tick():
+        pass
+    assert [j.jid for j in sch.failed] == [0] and [j.jid for j in sch.done] == [1]
+    bad = sch.failed[0]
+    assert bad.done_steps == 1 and bad.status & 1, (bad.done_steps, bad.status, bad.error)
+    assert "did not converge" in bad.error
+    good = sch.done[0]
+    x1, v1, status = alone(pm, tb, pb, 60)
+    assert status == 0 and good.status == 0
+    assert np.array_equal(good.final_positions, x1) and np.array_equal(good.final_velocities, v1)
+    for s in range(3):
+        assert not pool.occupied(s)

@@ -97,3 +97,65 @@ def test_oversized_job_waits_without_blocking_others():
     sch.submit(small)
     sch.tick()
     assert small.slot == 0 and huge.slot is None and list(sch.queue) == [huge]
+
+
+class FaultyPool(FakePool):
+    """A pool whose step latches a fault in chosen slots at a chosen step."""
+
+    def __init__(self, atom_caps, fault_at):
+        super().__init__(atom_caps)
+        self.fault_at = fault_at  # slot -> step count at which it halts
+        self.steps = {}
+        self.status = {}
+
+    def admit(self, slot, top, params):
+        super().admit(slot, top, params)
+        self.steps[slot] = 0
+        self.status[slot] = 0
+
+    def step(self, n, graph_steps):
+        super().step(n, graph_steps)
+        err = None
+        for slot in self.occupied:
+            if self.status[slot]:
+                continue
+            at = self.fault_at.get(slot)
+            if at is not None and self.steps[slot] + n >= at:
+                self.steps[slot] = at
+                self.status[slot] = 1
+                err = err or RuntimeError(f"constraint iteration did not converge (system {slot})")
+            else:
+                self.steps[slot] += n
+        if err is not None:
+            raise err
+
+    def stats(self, slot):
+        return {"status": self.status[slot], "steps_done": self.steps[slot]}
+
+
+def test_faulted_job_is_retired_and_others_continue():
+    pool = FaultyPool([5000, 5000], fault_at={0: 25})
+    sch = Scheduler(pool, graph_steps=10, clock=Clock())
+    bad, good, late = Job(0, top(1000), None, 100), Job(1, top(2000), None, 60), Job(2, top(1500), None, 20)
+    for j in (bad, good, late):
+        sch.submit(j)
+    while sch.tick():
+        pass
+    assert [j.jid for j in sch.failed] == [0] and bad.done_steps == 25 and bad.status == 1
+    assert "did not converge" in bad.error
+    assert {j.jid for j in sch.done} == {1, 2}  # the freed slot took the late job
+    assert late.slot == 0 and good.done_steps == 60
+    rep = sch.report(wall=1.0)
+    assert rep["faulted_jobs"] == [0] and rep["faults"][0]["at_step"] == 25
+    assert rep["atom_steps_per_s"] == pytest.approx(1000 * 25 + 2000 * 60 + 1500 * 20)
+
+
+def test_failure_naming_no_job_is_raised():
+    class Broken(FakePool):
+        def step(self, n, graph_steps):
+            raise RuntimeError("device lost")
+
+    sch = Scheduler(Broken([1000]), graph_steps=10, clock=Clock())
+    sch.submit(Job(0, top(100), None, 10))
+    with pytest.raises(RuntimeError, match="device lost"):
+        sch.tick()
