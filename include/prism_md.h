@@ -340,6 +340,25 @@ int prism_md_batch_retire(PrismMdBatch *b, int32_t slot);
 int prism_md_batch_slot_info(PrismMdBatch *b, int32_t slot, PrismMdSlotCapsV1 *caps, int32_t *occupied);
 int prism_md_batch_device_bytes(PrismMdBatch *b, uint64_t *bytes);
 
+/* ---- checkpoint / restore ---------------------------------------------------
+ * The dynamic state of a system -- positions, velocities, the positions and
+ * block boxes of the last pair-list build, the sorted atom order, the step
+ * counter and rebuild bookkeeping, the carried centre-of-mass velocity -- as a
+ * self-describing host buffer.  The Langevin noise is counter-based (Philox
+ * keyed by seed and stream_id, counter = atom and step), so no generator state
+ * is stored.  Restoring into any engine or batch slot that holds the same
+ * topology and run parameters (checked by digest; E_INVALID otherwise) rebuilds
+ * the same tile rows and carried force, and the trajectory continues bitwise
+ * as if it had not been interrupted.  save needs cap >= checkpoint_size.
+ * Replaces capture_process_checkpoint_fields / restore_process_checkpoint_fields
+ * (fused_engine.rs:24719, :25076). */
+int prism_md_checkpoint_size(PrismMdHandle *h, size_t *bytes);
+int prism_md_checkpoint_save(PrismMdHandle *h, void *buf, size_t cap, size_t *written);
+int prism_md_checkpoint_restore(PrismMdHandle *h, const void *buf, size_t len);
+int prism_md_batch_checkpoint_size(PrismMdBatch *b, int32_t system, size_t *bytes);
+int prism_md_batch_checkpoint_save(PrismMdBatch *b, int32_t system, void *buf, size_t cap, size_t *written);
+int prism_md_batch_checkpoint_restore(PrismMdBatch *b, int32_t system, const void *buf, size_t len);
+
 /* Sample the nonbonded kernel inside prism_md_step: with every > 0, HIP
  * events on the handle's stream bracket the nonbonded launch of every
  * every-th step from now on (at most 1024 samples per prism_md_step call).

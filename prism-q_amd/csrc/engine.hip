@@ -309,6 +309,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     P_.vals_a = dalloc<int>(A);
     P_.vals_b = dalloc<int>(A);
     P_.com_partial = dalloc<double>((size_t)W * 3);
+    P_.sweeps = dalloc<int>((size_t)W * 2);
     P_.counters = dalloc<EngineCounters>(1);
     d_count_ = dalloc<unsigned long long>(1);
     d_red_ = dalloc<double>((size_t)n_sys * 4);
@@ -349,6 +350,49 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
 
 // Lower one topology into slot s of the host mirror (padding after it) and
 // fill the slot's descriptor.  Nothing is uploaded here.
+// FNV-1a 64 over byte ranges: identity of a system for checkpoint restore.
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void *p, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    }
+    template <class T> void arr(const T *a, size_t n) {
+        add(&n, sizeof n);
+        if (a && n) add(a, n * sizeof(T));
+    }
+};
+
+// Everything of a topology except its positions.
+static uint64_t topology_digest(const PrismMdTopologyV1 *t) {
+    Fnv f;
+    const size_t n = (size_t)t->n_atoms;
+    f.arr(t->masses, n); f.arr(t->charges, n); f.arr(t->sigma, n); f.arr(t->epsilon, n);
+    f.arr(t->bond_atoms, 2 * (size_t)t->n_bonds); f.arr(t->bond_r0, t->n_bonds); f.arr(t->bond_k, t->n_bonds);
+    f.arr(t->angle_atoms, 3 * (size_t)t->n_angles); f.arr(t->angle_theta0, t->n_angles); f.arr(t->angle_k, t->n_angles);
+    f.arr(t->dihedral_atoms, 4 * (size_t)t->n_dihedrals); f.arr(t->dihedral_periodicity, t->n_dihedrals);
+    f.arr(t->dihedral_phase, t->n_dihedrals); f.arr(t->dihedral_k, t->n_dihedrals);
+    f.arr(t->exclusion_offsets, n + 1); f.arr(t->exclusion_atoms, (size_t)t->exclusion_offsets[n]);
+    f.arr(t->exception_atoms, 2 * (size_t)t->n_exceptions); f.arr(t->exception_charge_product, t->n_exceptions);
+    f.arr(t->exception_sigma, t->n_exceptions); f.arr(t->exception_epsilon, t->n_exceptions);
+    f.arr(t->cluster_central, t->n_clusters); f.arr(t->cluster_hydrogens, 3 * (size_t)t->n_clusters);
+    f.arr(t->cluster_bond_lengths, 3 * (size_t)t->n_clusters); f.arr(t->cluster_n_hydrogens, t->n_clusters);
+    f.arr(t->cluster_inv_mass_central, t->n_clusters); f.arr(t->cluster_inv_mass_h, t->n_clusters);
+    return f.h;
+}
+
+// The run parameters of a descriptor, without its arena placement.
+static uint64_t params_digest(const SysDesc &D) {
+    SysDesc d = D;
+    d.atom_begin = d.block_begin = d.wave_begin = 0;
+    d.active = 0;
+    d.noise_explicit = 0;
+    d.bond_begin = d.angle_begin = d.dih_begin = d.exc_begin = 0;
+    Fnv f;
+    f.add(&d, sizeof d);
+    return f.h;
+}
+
 int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     const Slot &L = slots_[s];
     const std::string over = fits(L.caps, t);
@@ -554,6 +598,8 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     D.exc_begin = L.exc_base;
     D.n_exc = t->n_exceptions;
     desc_[s] = D;
+    H.topo_digest = topology_digest(t);
+    H.params_digest = params_digest(D);
     sys_[s] = std::move(H);
     return PRISM_MD_OK;
 }
@@ -600,6 +646,11 @@ int Engine::upload_slot(int s) {
     S.rebuild_step = -1;
     S.fault_atom = -1;
     if ((rc = hipcheck(hipMemcpyAsync(P_.sst + s, &S, sizeof(S), hipMemcpyHostToDevice, stream_), "slot state")))
+        return rc;
+    const std::vector<int> zeros((size_t)L.wave_cap * 2, 0);
+    if ((rc = hipcheck(hipMemcpyAsync(P_.sweeps + (size_t)L.wave_base * 2, zeros.data(), zeros.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, stream_),
+                       "slot sweeps")))
         return rc;
     // the host staging buffers above must outlive the copies
     return hipcheck(hipStreamSynchronize(stream_), "slot upload");
@@ -666,6 +717,10 @@ int Engine::refresh(int only) {
                               "flag"))
             return rc;
     }
+    if (int rc = hipcheck(hipMemcpyAsync((char *)P_.counters + offsetof(EngineCounters, any_rebuild), ones, sizeof(int),
+                                         hipMemcpyHostToDevice, stream_),
+                          "flag"))
+        return rc;
     launch_sort(P_, stream_);
     launch_rows(P_, stream_);
     launch_consume(P_, -1, stream_);
@@ -673,17 +728,11 @@ int Engine::refresh(int only) {
     return hipcheck(hipGetLastError(), "refresh");
 }
 
-void Engine::full_force() {
-    launch_nonbonded(P_, false, stream_);
-    launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
-    launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
-    ++force_evals_;
-}
-
-void Engine::launch_step(const float *noise_n, const float *noise_np1) {
-    launch_first_half(P_, noise_n, stream_);
-    launch_sort(P_, stream_);
-    launch_rows(P_, stream_);
+// The step's force evaluation: nonbonded (sampled by the kernel timer),
+// bonded, then the fixed-order gather.  (Running the bonded terms on a forked
+// second stream beside the nonbonded kernel was measured slower: the fork and
+// join cost more than the overlap gains.)
+void Engine::launch_forces() {
     const bool sample = timing_every_ > 0 && timing_counter_++ % (uint64_t)timing_every_ == 0 &&
                         2 * ev_used_ + 1 < ev_pool_.size();
     if (sample) hipEventRecord(ev_pool_[2 * ev_used_], stream_);
@@ -691,6 +740,18 @@ void Engine::launch_step(const float *noise_n, const float *noise_np1) {
     if (sample) hipEventRecord(ev_pool_[2 * ev_used_++ + 1], stream_);
     launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
     launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
+}
+
+void Engine::full_force() {
+    launch_forces();
+    ++force_evals_;
+}
+
+void Engine::launch_step(const float *noise_n, const float *noise_np1) {
+    launch_first_half(P_, noise_n, stream_);
+    launch_sort(P_, stream_);
+    launch_rows(P_, stream_);
+    launch_forces();
     launch_second_half(P_, noise_np1, stream_);
     ++force_evals_;
 }
@@ -1078,8 +1139,13 @@ int Engine::stats(int s, PrismMdStatsV1 *out) {
     out->n_tiles = std::accumulate(rn.begin(), rn.end(), 0);
     out->n_blocks = H.n_blocks;
     out->status = S.status;
-    out->max_shake_sweeps = S.max_shake_iter;
-    out->max_rattle_sweeps = S.max_rattle_iter;
+    std::vector<int> sw((size_t)H.n_waves * 2);
+    if (!sw.empty())
+        if (int rc = d2h(sw.data(), P_.sweeps + (size_t)H.wave_begin * 2, sw.size() * sizeof(int))) return rc;
+    for (size_t k = 0; k < sw.size(); k += 2) {
+        out->max_shake_sweeps = std::max(out->max_shake_sweeps, sw[k]);
+        out->max_rattle_sweeps = std::max(out->max_rattle_sweeps, sw[k + 1]);
+    }
     return PRISM_MD_OK;
 }
 
@@ -1097,12 +1163,142 @@ int Engine::set_active(int s, bool active) {
         if (int rc = refresh(s)) return rc;
     } else {
         // drop its units: every active block re-claims, no row changes
-        const EngineCounters c{0u, 1, 0, 0};
+        EngineCounters c{};
+        c.any_dirty = 1;
         if (int rc = hipcheck(hipMemcpyAsync(P_.counters, &c, sizeof(c), hipMemcpyHostToDevice, stream_), "counters"))
             return rc;
         launch_rows(P_, stream_);
         launch_consume(P_, -1, stream_);
     }
+    return check_status();
+}
+
+// ---- checkpoint -------------------------------------------------------------
+// Analogue of capture_process_checkpoint_fields / restore_process_checkpoint_fields
+// (fused_engine.rs:24719, :25076): the engine's mutable per-system state.
+// Noise is counter-based (Philox keyed by seed and stream, counter = atom and
+// step), so the step counter is the whole RNG state.
+namespace {
+constexpr char CKPT_MAGIC[8] = {'P', 'R', 'I', 'S', 'M', 'C', 'K', '1'};
+struct CheckpointHeader {
+    char magic[8];
+    uint32_t version, header_bytes;
+    int32_t n_atoms, n_blocks;
+    uint64_t topo_digest, params_digest, payload_bytes;
+    SysState state;
+};
+size_t payload_bytes(int n, int nb) {
+    return 3 * (size_t)n * sizeof(float4) + (size_t)nb * BLOCK * sizeof(int) + 2 * (size_t)nb * sizeof(float4);
+}
+}  // namespace
+
+size_t Engine::checkpoint_bytes(int s) const {
+    if (s < 0 || s >= (int)sys_.size()) return 0;
+    return sizeof(CheckpointHeader) + payload_bytes(sys_[s].n_atoms, sys_[s].n_blocks);
+}
+
+int Engine::checkpoint_save(int s, void *buf, size_t cap, size_t *written) {
+    if (s < 0 || s >= (int)sys_.size() || !buf) return fail(PRISM_MD_E_INVALID, "checkpoint: system index or buffer");
+    if (!desc_[s].active) return fail(PRISM_MD_E_INVALID, "checkpoint: slot holds no running system");
+    const HostSys &H = sys_[s];
+    const int n = H.n_atoms, nb = H.n_blocks;
+    const size_t need = checkpoint_bytes(s);
+    if (cap < need) return fail(PRISM_MD_E_CAPACITY, "checkpoint: buffer smaller than checkpoint_size");
+    if (int rc = hipcheck(hipStreamSynchronize(stream_), "checkpoint sync")) return rc;
+    CheckpointHeader hd{};
+    std::memcpy(hd.magic, CKPT_MAGIC, 8);
+    hd.version = 1;
+    hd.header_bytes = sizeof(CheckpointHeader);
+    hd.n_atoms = n;
+    hd.n_blocks = nb;
+    hd.topo_digest = H.topo_digest;
+    hd.params_digest = H.params_digest;
+    hd.payload_bytes = payload_bytes(n, nb);
+    if (int rc = d2h(&hd.state, P_.sst + s, sizeof(SysState))) return rc;
+    char *out = static_cast<char *>(buf);
+    std::memcpy(out, &hd, sizeof hd);
+    char *p = out + sizeof hd;
+    for (const float4 *src : {P_.x, P_.v, P_.x_build}) {
+        if (int rc = d2h(p, src + H.atom_begin, (size_t)n * sizeof(float4))) return rc;
+        p += (size_t)n * sizeof(float4);
+    }
+    std::vector<int> perm((size_t)nb * BLOCK);
+    if (int rc = d2h(perm.data(), P_.perm + (size_t)H.block_begin * BLOCK, perm.size() * sizeof(int))) return rc;
+    for (int &a : perm) a = a >= 0 ? a - H.atom_begin : -1;  // position independent
+    std::memcpy(p, perm.data(), perm.size() * sizeof(int));
+    p += perm.size() * sizeof(int);
+    for (const float4 *src : {P_.bb_c, P_.bb_h}) {
+        if (int rc = d2h(p, src + H.block_begin, (size_t)nb * sizeof(float4))) return rc;
+        p += (size_t)nb * sizeof(float4);
+    }
+    if (written) *written = need;
+    return PRISM_MD_OK;
+}
+
+int Engine::checkpoint_restore(int s, const void *buf, size_t len) {
+    if (s < 0 || s >= (int)sys_.size() || !buf) return fail(PRISM_MD_E_INVALID, "restore: system index or buffer");
+    if (!desc_[s].active) return fail(PRISM_MD_E_INVALID, "restore: slot holds no running system");
+    const HostSys &H = sys_[s];
+    const int n = H.n_atoms, nb = H.n_blocks;
+    CheckpointHeader hd;
+    if (len < sizeof hd) return fail(PRISM_MD_E_INVALID, "restore: buffer shorter than a checkpoint header");
+    std::memcpy(&hd, buf, sizeof hd);
+    if (std::memcmp(hd.magic, CKPT_MAGIC, 8) != 0 || hd.version != 1 || hd.header_bytes != sizeof hd)
+        return fail(PRISM_MD_E_INVALID, "restore: not a prism_md checkpoint of this version");
+    if (hd.n_atoms != n || hd.n_blocks != nb || hd.topo_digest != H.topo_digest)
+        return fail(PRISM_MD_E_INVALID, "restore: checkpoint of a different topology");
+    if (hd.params_digest != H.params_digest)
+        return fail(PRISM_MD_E_INVALID, "restore: checkpoint taken with different run parameters");
+    if (hd.payload_bytes != payload_bytes(n, nb) || len != sizeof hd + hd.payload_bytes)
+        return fail(PRISM_MD_E_INVALID, "restore: checkpoint length does not match its header");
+    const char *p = static_cast<const char *>(buf) + sizeof hd;
+    std::vector<float4> x3(3 * (size_t)n);
+    std::memcpy(x3.data(), p, x3.size() * sizeof(float4));
+    p += x3.size() * sizeof(float4);
+    std::vector<int> perm((size_t)nb * BLOCK);
+    std::memcpy(perm.data(), p, perm.size() * sizeof(int));
+    p += perm.size() * sizeof(int);
+    std::vector<float4> boxes(2 * (size_t)nb);
+    std::memcpy(boxes.data(), p, boxes.size() * sizeof(float4));
+    std::vector<char> seen(n, 0);
+    for (int &a : perm) {  // a permutation of the system's atoms plus padding
+        if (a < -1 || a >= n || (a >= 0 && seen[a]++)) return fail(PRISM_MD_E_INVALID, "restore: corrupt slot order");
+        if (a >= 0) a += H.atom_begin;
+    }
+    for (const float4 &q : x3)
+        if (!std::isfinite(q.x) || !std::isfinite(q.y) || !std::isfinite(q.z))
+            return fail(PRISM_MD_E_NONFINITE, "restore: non-finite state");
+    int rc = PRISM_MD_OK;
+    const size_t a0 = H.atom_begin;
+    if ((rc = hipcheck(hipMemcpyAsync(P_.x + a0, x3.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice, stream_), "restore x")) ||
+        (rc = hipcheck(hipMemcpyAsync(P_.v + a0, x3.data() + n, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, stream_), "restore v")) ||
+        (rc = hipcheck(hipMemcpyAsync(P_.x_build + a0, x3.data() + 2 * (size_t)n, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, stream_), "restore x_build")) ||
+        (rc = hipcheck(hipMemcpyAsync(P_.perm + (size_t)H.block_begin * BLOCK, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "restore perm")) ||
+        (rc = hipcheck(hipMemcpyAsync(P_.bb_c + H.block_begin, boxes.data(), (size_t)nb * sizeof(float4), hipMemcpyHostToDevice, stream_), "restore boxes")) ||
+        (rc = hipcheck(hipMemcpyAsync(P_.bb_h + H.block_begin, boxes.data() + nb, (size_t)nb * sizeof(float4), hipMemcpyHostToDevice, stream_), "restore boxes")))
+        return rc;
+    launch_restore_slots(P_, s, nb * BLOCK, stream_);
+    // rows and exclusion masks of the restored blocks from the restored boxes
+    // (flags make this system's rows rebuild with masks); every block re-claims
+    // its units
+    SysState st = hd.state;
+    st.rebuild_flag = 1;
+    st.resort_flag = 1;
+    EngineCounters c{};
+    c.any_dirty = 1;
+    if ((rc = hipcheck(hipMemcpyAsync(P_.sst + s, &st, sizeof st, hipMemcpyHostToDevice, stream_), "restore state")) ||
+        (rc = hipcheck(hipMemcpyAsync(P_.counters, &c, sizeof c, hipMemcpyHostToDevice, stream_), "restore counters")))
+        return rc;
+    launch_rows(P_, stream_);
+    SysState after;
+    if ((rc = d2h(&after, P_.sst + s, sizeof after))) return rc;
+    if (after.status & (ST_ROW_OVERFLOW | ST_MASK_OVERFLOW)) return fail(PRISM_MD_E_CAPACITY, "restore: tile rows overflow");
+    // the saved bookkeeping replaces the rebuild flags; the unit list claimed
+    // above stays (any_dirty is cleared at the end of the next step)
+    if ((rc = hipcheck(hipMemcpyAsync(P_.sst + s, &hd.state, sizeof hd.state, hipMemcpyHostToDevice, stream_), "restore state")))
+        return rc;
+    full_force();  // the carried force F(x)
+    if ((rc = hipcheck(hipGetLastError(), "restore"))) return rc;
     return check_status();
 }
 
@@ -1312,6 +1508,29 @@ int prism_md_batch_device_bytes(PrismMdBatch *b, uint64_t *bytes) {
     if (!b || !bytes) return PRISM_MD_E_INVALID;
     *bytes = b->eng.device_bytes();
     return PRISM_MD_OK;
+}
+
+int prism_md_checkpoint_size(PrismMdHandle *h, size_t *bytes) {
+    if (!h || !bytes) return PRISM_MD_E_INVALID;
+    *bytes = h->eng.checkpoint_bytes(0);
+    return PRISM_MD_OK;
+}
+int prism_md_checkpoint_save(PrismMdHandle *h, void *buf, size_t cap, size_t *written) {
+    return h ? h->eng.checkpoint_save(0, buf, cap, written) : PRISM_MD_E_INVALID;
+}
+int prism_md_checkpoint_restore(PrismMdHandle *h, const void *buf, size_t len) {
+    return h ? h->eng.checkpoint_restore(0, buf, len) : PRISM_MD_E_INVALID;
+}
+int prism_md_batch_checkpoint_size(PrismMdBatch *b, int32_t system, size_t *bytes) {
+    if (!b || !bytes || system < 0 || system >= b->eng.n_systems()) return PRISM_MD_E_INVALID;
+    *bytes = b->eng.checkpoint_bytes(system);
+    return PRISM_MD_OK;
+}
+int prism_md_batch_checkpoint_save(PrismMdBatch *b, int32_t system, void *buf, size_t cap, size_t *written) {
+    return b ? b->eng.checkpoint_save(system, buf, cap, written) : PRISM_MD_E_INVALID;
+}
+int prism_md_batch_checkpoint_restore(PrismMdBatch *b, int32_t system, const void *buf, size_t len) {
+    return b ? b->eng.checkpoint_restore(system, buf, len) : PRISM_MD_E_INVALID;
 }
 
 int prism_md_time_nonbonded(PrismMdHandle *h, int iters, double *mean_ms) {

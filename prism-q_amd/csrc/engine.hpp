@@ -58,6 +58,13 @@ class Engine {
     int minimize(int max_iter, double f_tol, double *e_final, double *f_max);
     int stats(int s, PrismMdStatsV1 *out);
     int set_active(int s, bool active);
+    // Dynamic state of system s as a self-describing blob (positions,
+    // velocities, build-time positions, sorted order, block boxes, step and
+    // rebuild bookkeeping); restoring it into an engine holding the same
+    // topology and parameters continues the trajectory bitwise.
+    size_t checkpoint_bytes(int s) const;
+    int checkpoint_save(int s, void *buf, size_t cap, size_t *written);
+    int checkpoint_restore(int s, const void *buf, size_t len);
     int check_status();           // synchronise, read device status, map to rc
     const char *last_error() const { return err_.c_str(); }
     hipStream_t stream() const { return stream_; }
@@ -89,6 +96,7 @@ class Engine {
         RefGrid grid;
         int grid_ok;
         double dof;
+        uint64_t topo_digest, params_digest;  // identity checked by checkpoint_restore
     };
     std::vector<HostSys> sys_;
     std::vector<SysDesc> desc_;
@@ -128,6 +136,7 @@ class Engine {
     DevPtrs P_{};
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
+    void launch_forces();  // nonbonded, bonded, gather
     std::vector<void *> allocs_;
     int timing_every_ = 0;        // sampled nonbonded timing (kernel_timing)
     std::vector<hipEvent_t> ev_pool_;  // pairs: [2k] before, [2k+1] after

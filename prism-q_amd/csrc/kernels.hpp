@@ -58,6 +58,8 @@ struct DevPtrs {
     unsigned int *keys_a, *keys_b;
     int *vals_a, *vals_b;
     double *com_partial;            // [waves][3]
+    int *sweeps;                    // [waves][2]: most SHAKE / RATTLE sweeps so far, per group wave
+                                    // (written only by that wave; reduced on the host)
     EngineCounters *counters;
 };
 
@@ -73,6 +75,8 @@ struct RefGrid {                    // reference cell grid of the pair-list pari
 void launch_sort(const DevPtrs &P, hipStream_t s);
 void launch_rows(const DevPtrs &P, hipStream_t s);
 void launch_consume(const DevPtrs &P, int step_offset, hipStream_t s);
+// slot_of / xq / lj of system s's n_slots sorted slots from its uploaded perm
+void launch_restore_slots(const DevPtrs &P, int s, int n_slots, hipStream_t st);
 void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s);
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);

@@ -72,6 +72,32 @@ __device__ inline void refresh_block(const DevPtrs &P, int b, int lane) {
     }
 }
 
+// Sorted slot `slot` (the i-th of its system) holding arena atom a (-1:
+// padding): inverse permutation, position with the scaled charge, and the
+// Lorentz-Berthelot halves (sigma/2, sqrt(24 eps)).  Padding slots sit far
+// apart outside every cutoff.
+__device__ inline void fill_slot(const DevPtrs &P, int slot, int i, int a) {
+    if (a >= 0) {
+        const float qscale = (float)sqrt(COULOMB);
+        const float4 p = P.x[a];
+        P.slot_of[a] = slot;
+        P.xq[slot] = make_float4(p.x, p.y, p.z, P.charge[a] * qscale);
+        P.lj[slot] = make_float2(0.5f * P.sigma[a], sqrtf(24.0f * P.eps[a]));
+    } else {
+        P.xq[slot] = make_float4(DUMMY_FAR + 64.0f * (float)i, DUMMY_FAR, DUMMY_FAR, 0.0f);
+        P.lj[slot] = make_float2(0.0f, 0.0f);
+    }
+}
+
+// Slots of system s rebuilt from an uploaded permutation (checkpoint restore).
+__global__ __launch_bounds__(256) void k_restore_slots(DevPtrs P, int s) {
+    const SysDesc &D = P.sys[s];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.n_blocks * BLOCK) return;
+    const int slot = D.block_begin * BLOCK + i;
+    fill_slot(P, slot, i, P.perm[slot]);
+}
+
 // Workgroups [0, n_sys): one per system, a full re-sort when the system's
 // rebuild is due and resort_due() holds.  Workgroups from n_sys on: one wave
 // per block, a box refresh of blocks whose system rebuilds without re-sorting.
@@ -83,6 +109,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
     __shared__ unsigned int running[256];
     __shared__ unsigned int wave_cnt[SORT_WAVES][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (P.counters->any_rebuild == 0) return;  // nothing to rebuild this step: one load
     if ((int)blockIdx.x >= P.n_sys) {
         const int b = ((int)blockIdx.x - P.n_sys) * SORT_WAVES + wave;
         if (b >= P.n_blocks_arena) return;
@@ -206,21 +233,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
 
     // ---- gather into sorted slots
     const int slot0 = D.block_begin * BLOCK;
-    const float qscale = (float)sqrt(COULOMB);
     for (int i = tid; i < D.n_blocks * BLOCK; i += SORT_THREADS) {
-        const int slot = slot0 + i;
-        if (i < n) {
-            const int a = a0 + vin[i];
-            const float4 p = P.x[a];
-            P.perm[slot] = a;
-            P.slot_of[a] = slot;
-            P.xq[slot] = make_float4(p.x, p.y, p.z, P.charge[a] * qscale);
-            P.lj[slot] = make_float2(0.5f * P.sigma[a], sqrtf(24.0f * P.eps[a]));
-        } else {
-            P.perm[slot] = -1;
-            P.xq[slot] = make_float4(DUMMY_FAR + 64.0f * (float)i, DUMMY_FAR, DUMMY_FAR, 0.0f);
-            P.lj[slot] = make_float2(0.0f, 0.0f);
-        }
+        const int a = i < n ? a0 + vin[i] : -1;
+        P.perm[slot0 + i] = a;
+        fill_slot(P, slot0 + i, i, a);
     }
     __syncthreads();
 
@@ -385,7 +401,10 @@ __global__ __launch_bounds__(256) void k_consume(DevPtrs P, int step_offset) {
         S.rebuild_step = (long long)S.step + step_offset;
         S.n_rebuilds += 1;
     }
-    if (threadIdx.x == 0) P.counters->any_dirty = 0;
+    if (threadIdx.x == 0) {
+        P.counters->any_dirty = 0;
+        P.counters->any_rebuild = 0;
+    }
 }
 
 }  // namespace
@@ -398,6 +417,10 @@ void launch_sort(const DevPtrs &P, hipStream_t st) {
 void launch_rows(const DevPtrs &P, hipStream_t st) {
     const int grid = (P.n_blocks_arena + ROW_WAVES - 1) / ROW_WAVES;
     hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), 0, st, P);
+}
+
+void launch_restore_slots(const DevPtrs &P, int s, int n_slots, hipStream_t st) {
+    hipLaunchKernelGGL(k_restore_slots, dim3((n_slots + 255) / 256), dim3(256), 0, st, P, s);
 }
 
 void launch_consume(const DevPtrs &P, int step_offset, hipStream_t st) {

@@ -83,8 +83,11 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     float4 *jpos = jpos_s[wave];
     float4 *jpar = jpar_s[wave];
     const int n_units = P.counters->n_units;
+    const int n_waves = gridDim.x * NB_WAVES;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int u = blockIdx.x * NB_WAVES + wave; u < n_units; u += gridDim.x * NB_WAVES) {
+    // Static round-robin over the unit list.  (A shared claim counter was
+    // measured far slower: device-scope atomics on one address serialise.)
+    for (int u = blockIdx.x * NB_WAVES + wave; u < n_units; u += n_waves) {
         const int code = P.units[u];
         const int b = code >> 8, q = (code >> 6) & 3, c = code & 63;
         const int s = P.block_sys[b];
@@ -208,8 +211,7 @@ __device__ inline void put(const DevPtrs &P, int slot, double x, double y, doubl
 // cancellation of r - r0 and theta - theta0 near equilibrium that limits an
 // FP32 evaluation to ~1e-5 relative.
 template <bool ENERGY>
-__global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
     int idx = tid;
     double energy = 0.0;
     if (idx < P.n_bonds) {
@@ -318,6 +320,11 @@ __global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
         return;
     }
     if (ENERGY) P.term_e[tid] = energy;
+}
+
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
+    bonded_term<ENERGY>(P, mask, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __device__ inline long long wave_xor_ll(long long v, int m) {
@@ -485,13 +492,33 @@ __global__ __launch_bounds__(256) void k_pairs(DevPtrs P, int mode, RefGrid g, i
 
 }  // namespace
 
-void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st) {
-    // persistent grid: enough waves to cover the unit list of a typical row
-    // (80 tiles -> 40 units per block), capped at 32 waves per CU
+// Workgroups of a nonbonded kernel variant that the device holds at once
+// (occupancy x compute units), queried once per device.
+template <bool ENERGY> static int resident_blocks() {
+    static int cache[64];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int &c = cache[dev & 63];
+    if (c == 0) {
+        int per_cu = 0, cus = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbonded<ENERGY>, 64 * NB_WAVES, 0);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        c = max(1, per_cu) * max(1, cus);
+    }
+    return c;
+}
+
+template <bool ENERGY> static void launch_nb(const DevPtrs &P, hipStream_t st) {
+    // persistent grid: every wave resident at once (no second round of
+    // workgroups), but no more waves than a typical unit list needs
     const int want = (P.n_blocks_arena * QUARTERS * 12 + NB_WAVES - 1) / NB_WAVES;
-    const int grid = max(1, min(want, 256 * 32 / NB_WAVES));
-    if (energy) hipLaunchKernelGGL(k_nonbonded<true>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
-    else hipLaunchKernelGGL(k_nonbonded<false>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
+    const int grid = max(1, min(want, resident_blocks<ENERGY>()));
+    hipLaunchKernelGGL(k_nonbonded<ENERGY>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
+}
+
+void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st) {
+    if (energy) launch_nb<true>(P, st);
+    else launch_nb<false>(P, st);
 }
 
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t st) {

@@ -277,10 +277,12 @@ __device__ inline void note_sweeps(SysState &S, int sweeps, int kind, int atom, 
     if (sweeps < 0) record_fault(S, kind, atom, step);
     else most = max(most, sweeps);
 }
-// One atomicMax per wave for the sweep statistics (called in converged control flow).
-__device__ inline void publish_sweeps(int *dst, int most) {
+// Sweep statistics: the wave's maximum raises the running maximum `prev`
+// (read at kernel start) of this wave's own slot; no atomics, called in
+// converged control flow.
+__device__ inline void publish_sweeps(int *slot, int prev, int most) {
     for (int o = 32; o > 0; o >>= 1) most = max(most, __shfl_xor(most, o));
-    if ((threadIdx.x & 63) == 0 && most > 0) atomicMax(dst, most);
+    if ((threadIdx.x & 63) == 0 && most > prev) *slot = most;
 }
 
 __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *noise) {
@@ -297,6 +299,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
     float T, gamma;
     schedule(D, step + 1, T, gamma);
     const bool com = D.com_every > 0 && step > 0 && step % (unsigned long long)D.com_every == 0;
+    const int prev_sh = P.sweeps[2 * gw], prev_ra = P.sweeps[2 * gw + 1];
     GroupState g;
     GroupParam gp;
     bool moved = false;
@@ -331,11 +334,14 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
             moved |= dx * dx + dy * dy + dz * dz > D.half_skin2;
         }
     }
-    publish_sweeps(&S.max_shake_iter, most_sh);
-    publish_sweeps(&S.max_rattle_iter, most_ra);
+    publish_sweeps(P.sweeps + 2 * gw, prev_sh, most_sh);
+    publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
     const bool due = (long long)step - S.rebuild_step >= (long long)D.rebuild_every;
     if (__ballot(moved) != 0ull || due) {
-        if (lane == 0) S.rebuild_flag = 1;
+        if (lane == 0) {
+            S.rebuild_flag = 1;
+            P.counters->any_rebuild = 1;
+        }
     }
 }
 
@@ -351,6 +357,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         float T, gamma;
         schedule(D, step + 1, T, gamma);
         const bool next_com = D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0;
+        const int prev_ra = P.sweeps[2 * gw + 1];
         GroupState g;
         GroupParam gp;
         double mv[3] = {0.0, 0.0, 0.0};
@@ -375,7 +382,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
                 for (int c = 0; c < 3; ++c) mv[c] += (double)g.m[k] * g.v[k][c];
             }
         }
-        publish_sweeps(&S.max_rattle_iter, most_ra);
+        publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
         if (next_com) {
             for (int c = 0; c < 3; ++c) mv[c] = wave_sum(mv[c]);
             if (lane == 0)
@@ -415,6 +422,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
     if (threadIdx.x == 0) {
         P.counters->done_blocks = 0;
         P.counters->any_dirty = 0;
+        P.counters->any_rebuild = 0;
     }
 }
 

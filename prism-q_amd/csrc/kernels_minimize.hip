@@ -74,7 +74,10 @@ __global__ __launch_bounds__(MT) void k_fire_update(DevPtrs P, float4 *vmin, flo
     *(float3 *)&P.xq[P.slot_of[a]] = make_float3(x.x, x.y, x.z);
     const float4 b = P.x_build[a];
     const float dx = x.x - b.x, dy = x.y - b.y, dz = x.z - b.z;
-    if (dx * dx + dy * dy + dz * dz > P.sys[s].half_skin2) P.sst[s].rebuild_flag = 1;
+    if (dx * dx + dy * dy + dz * dz > P.sys[s].half_skin2) {
+        P.sst[s].rebuild_flag = 1;
+        P.counters->any_rebuild = 1;
+    }
 }
 
 // Put every hydrogen at exactly its constraint length from its heavy atom,

@@ -88,8 +88,7 @@ struct SysState {
     int fault_atom;                 // first atom of the first failing group (-1: none)
     int fault_kind;                 // FAULT_* of that failure
     unsigned long long fault_step;
-    int max_shake_iter;             // most SHAKE sweeps any group needed
-    int max_rattle_iter;            // most RATTLE sweeps any group needed
+    int pad_sweeps[2];
     int halted;                     // set at the end of a step that latched an ST_HALT bit: the
                                     // integrator leaves the system as it was after that step
     int pad_state;
@@ -117,7 +116,9 @@ struct EngineCounters {
     unsigned int done_blocks;       // last-block handshake of the integrator
     int any_dirty;                  // some system rebuilds its rows this step
     int n_units;                    // nonbonded work units claimed by the row builder
-    int pad;
+    int any_rebuild;                // some system raised its rebuild flag (cleared with any_dirty);
+                                    // the rebuild kernels return at once while it is 0
+    int pad[4];
 };
 
 struct Group {                      // constraint group

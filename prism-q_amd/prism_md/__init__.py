@@ -148,6 +148,12 @@ def md_lib():
             "prism_md_last_error": ([H], C.c_char_p),
             "prism_md_count_pairs": ([H, C.POINTER(C.c_uint64)], C.c_int),
             "prism_md_time_nonbonded": ([H, C.c_int, C.POINTER(C.c_double)], C.c_int),
+            "prism_md_checkpoint_size": ([H, C.POINTER(C.c_size_t)], C.c_int),
+            "prism_md_checkpoint_save": ([H, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
+            "prism_md_checkpoint_restore": ([H, C.c_void_p, C.c_size_t], C.c_int),
+            "prism_md_batch_checkpoint_size": ([H, C.c_int32, C.POINTER(C.c_size_t)], C.c_int),
+            "prism_md_batch_checkpoint_save": ([H, C.c_int32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
+            "prism_md_batch_checkpoint_restore": ([H, C.c_int32, C.c_void_p, C.c_size_t], C.c_int),
             "prism_md_kernel_timing": ([H, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)], C.c_int),
             "prism_md_batch_create": ([C.POINTER(C.POINTER(TopologyV1)), C.POINTER(ParamsV1), C.c_int32, C.c_void_p,
                                        C.POINTER(H)], C.c_int),
@@ -471,6 +477,20 @@ class FusedEngine:
         self._check(self.lib.prism_md_kernel_timing(self.h, every, C.byref(ms), C.byref(n)), "kernel_timing")
         return float(ms.value), int(n.value)
 
+    def checkpoint(self) -> bytes:
+        """The system's dynamic state (prism_md_checkpoint_save): restoring it
+        into an engine of the same topology and parameters continues the
+        trajectory bitwise."""
+        n = C.c_size_t()
+        self._check(self.lib.prism_md_checkpoint_size(self.h, C.byref(n)), "checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        w = C.c_size_t()
+        self._check(self.lib.prism_md_checkpoint_save(self.h, buf, n, C.byref(w)), "checkpoint_save")
+        return buf.raw[:w.value]
+
+    def restore(self, blob: bytes):
+        self._check(self.lib.prism_md_checkpoint_restore(self.h, blob, len(blob)), "checkpoint_restore")
+
     def time_nonbonded(self, iters: int = 20) -> float:
         ms = C.c_double()
         self._check(self.lib.prism_md_time_nonbonded(self.h, iters, C.byref(ms)), "time_nonbonded")
@@ -570,6 +590,17 @@ class Ensemble:
         out = (C.c_double * 3)()
         self._check(self.lib.prism_md_batch_energies(self.h, s, out), "batch_energies")
         return {"potential": out[0], "kinetic": out[1], "temperature": out[2]}
+
+    def checkpoint(self, s: int) -> bytes:
+        n = C.c_size_t()
+        self._check(self.lib.prism_md_batch_checkpoint_size(self.h, s, C.byref(n)), "batch_checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        w = C.c_size_t()
+        self._check(self.lib.prism_md_batch_checkpoint_save(self.h, s, buf, n, C.byref(w)), "batch_checkpoint_save")
+        return buf.raw[:w.value]
+
+    def restore(self, s: int, blob: bytes):
+        self._check(self.lib.prism_md_batch_checkpoint_restore(self.h, s, blob, len(blob)), "batch_checkpoint_restore")
 
     def stats(self, s: int) -> dict:
         st = StatsV1()
