@@ -25,7 +25,13 @@ One JSON line on stdout:
                and the same systems run one at a time on the plain path;
   serving      continuous batching of Poisson-arriving jobs admitted into a
                slot pool sized to an HBM budget: p50/p99 per-step latency,
-               aggregate atom-steps/s, jobs done, admission wait.
+               aggregate atom-steps/s, jobs done, admission wait;
+  large_system the same generator at 250,000 atoms in one globule: ns/day and
+               the nonbonded FP32 rate at that size;
+  nve          energy drift of the 2,000-atom chain over 100,000 steps without
+               a thermostat (1 fs);
+  cpu_baseline.small_chain_full  the 2,000-atom chain, 1,000 steps at 2 fs,
+               through the FP64 oracle in full.
 """
 from __future__ import annotations
 
@@ -72,6 +78,83 @@ def cpu_baseline(pm, top, x, v, params, target_s):
                       f"{sps:.3f} steps/s, OpenMP threads={threads}"}
 
 
+def cpu_chain_c1(pm, n_atoms=2000, steps=1000):
+    """The small-chain configuration in full on the host: the FP64 oracle
+    runs 1,000 Langevin steps of the 2,000-atom chain at 2 fs, 300 K,
+    gamma 1/ps, SHAKE/RATTLE on X-H bonds."""
+    import numpy as np
+    from oracle import pyoracle as O
+    top = pm.Topology.generate(n_atoms, seed=GEN_SEED - 1)  # generator seed of the small-chain configuration
+    p = pm.default_params(dt_ps=0.002, gamma_ps=1.0, equilibration_steps=0)
+    x = top["positions"].reshape(-1, 3).astype(np.float64)
+    v = pm.maxwell_boltzmann(top["masses"], 300.0, 11).astype(np.float64)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    t0 = time.perf_counter()
+    O.run(top, p, x, v, 0, steps)
+    wall = time.perf_counter() - t0
+    sps = steps / wall
+    return {"value": round(sps * 0.002e-3 * 86400.0, 4), "unit": "ns/day", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the {n_atoms}-atom chain at 2 fs through the FP64 oracle, "
+                      f"{wall:.1f} s wall, {sps:.1f} steps/s, OpenMP threads={threads}"}
+
+
+def nve_leg(pm, n_atoms, steps, dt_ps, every):
+    """Energy drift without a thermostat (gamma = 0: exact kicks, no clamp):
+    the minimised chain from 300 K Maxwell-Boltzmann velocities, total energy
+    sampled every `every` steps; drift = least-squares slope."""
+    import numpy as np
+    top = pm.Topology.generate(n_atoms, seed=7)
+    p = pm.default_params(dt_ps=dt_ps, gamma_ps=0.0, equilibration_steps=0, com_remove_every=0)
+    t0 = time.perf_counter()
+    with pm.FusedEngine(top, p) as eng:
+        eng.minimize(2000, 1.0)
+        eng.set_state(v=pm.maxwell_boltzmann(top["masses"], 300.0, 3))
+        ts, es, ks = [], [], []
+        for k in range(steps // every + 1):
+            if k:
+                eng.step(every)
+            en = eng.energies()
+            ts.append(k * every * dt_ps)
+            es.append(en["potential"] + en["kinetic"])
+            ks.append(en["kinetic"])
+    t = np.asarray(ts)
+    e = np.asarray(es)
+    slope = float(np.polyfit(t, e, 1)[0])
+    return {"atoms": n_atoms, "steps": steps, "dt_ps": dt_ps, "sample_every": every,
+            "drift_kcal_mol_per_ps": round(slope, 4),
+            "drift_kcal_mol_per_ns_per_atom": round(slope * 1e3 / n_atoms, 5),
+            "total_energy_change_kcal_mol": round(float(e[-1] - e[0]), 3),
+            "mean_kinetic_kcal_mol": round(float(np.mean(ks)), 3),
+            "max_abs_deviation_kcal_mol": round(float(np.max(np.abs(e - e[0]))), 3),
+            "wall_s": round(time.perf_counter() - t0, 1)}
+
+
+def large_leg(pm, n_atoms, steps, warmup, sample_every):
+    """One large compact globule (same generator): ns/day and the nonbonded
+    roofline fraction at that size."""
+    t0 = time.perf_counter()
+    top = pm.Topology.generate(n_atoms, seed=GEN_SEED + 1)
+    params = pm.default_params()
+    with pm.FusedEngine(top, params) as eng:
+        eng.minimize(1000, 1.0)
+        setup = time.perf_counter() - t0
+        eng.step(warmup)
+        eng.kernel_timing(sample_every)
+        t1 = time.perf_counter()
+        eng.step(steps)
+        wall = time.perf_counter() - t1
+        nb_ms, nb_n = eng.kernel_timing(0)
+        pairs = eng.count_pairs()
+        st = eng.stats()
+    sps = steps / wall
+    achieved = FLOP_PER_PAIR * pairs / (nb_ms * 1e-3) / 1e12
+    return {"atoms": n_atoms, "steps": steps, "warmup": warmup, "ns_per_day": round(sps * params.dt_ps * 1e-3 * 86400.0, 2),
+            "ms_per_step": round(wall * 1e3 / steps, 4), "atom_steps_per_s": round(sps * n_atoms, 1),
+            "unique_pairs_in_cutoff": pairs, "nonbonded_mean_launch_ms": round(nb_ms, 4),
+            "nonbonded_tflops": round(achieved, 3), "nonbonded_frac_of_157_3": round(achieved / PEAK_FP32_TFLOPS, 5),
+            "status": st["status"], "setup_s": round(setup, 1)}
+
+
 def copy_calibration(mib: int, reps: int = 5):
     """Device-to-device copies of a known size through the HIP runtime, so a
     counter pass can scale FETCH_SIZE / WRITE_SIZE: each copy moves exactly
@@ -90,19 +173,28 @@ def copy_calibration(mib: int, reps: int = 5):
     log(f"copy calibration: {reps} device-to-device copies of {mib} MiB")
 
 
-def ensemble_leg(pm, n_systems, steps, graph_steps, seed=0x50524953 + 3):
+def ensemble_leg(pm, n_systems, steps, graph_steps, tune_buckets, seed=0x50524953 + 3):
     """Batched ensemble (one launch per phase over all systems, captured
     hipGraph) against the same systems run one at a time on the plain path.
     Sizes log-uniform in [500, 20000] atoms, temperatures uniform in
-    [280, 340] K, 4 fs, per-system generator and noise seeds."""
+    [280, 340] K, 4 fs, per-system generator and noise seeds.  The sort-cell
+    size and re-sort interval of each system come from the autotuner's
+    choice for its size bucket (prism_md.autotune)."""
     import numpy as np
+    from prism_md import autotune
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(500), np.log(20000), n_systems)).astype(int)
     temps = rng.uniform(280.0, 340.0, n_systems)
     t0 = time.perf_counter()
+    table = autotune.tune(pm, autotune.log_buckets(500, 20000, tune_buckets), graph_steps=graph_steps) \
+        if tune_buckets > 0 else []
+    log(f"autotune: {[(e['lo'], e['hi'], e['sort_cell_A'], e['resort_every'], e['us_per_step']) for e in table]} "
+        f"in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
     tops = [pm.Topology.generate(int(n), seed=seed + 1 + k) for k, n in enumerate(sizes)]
-    params = [pm.default_params(seed=seed + 1000 + k, stream_id=k, temperature_K=float(T), end_temperature_K=float(T))
-              for k, T in enumerate(temps)]
+    params = [pm.default_params(seed=seed + 1000 + k, stream_id=k, temperature_K=float(T), end_temperature_K=float(T),
+                                **(autotune.choice_for(table, int(n)) if table else {}))
+              for k, (T, n) in enumerate(zip(temps, sizes))]
     total = int(sizes.sum())
     log(f"ensemble: {n_systems} systems, {total} atoms generated in {time.perf_counter() - t0:.1f} s")
     ens = pm.Ensemble(tops, params)
@@ -164,6 +256,7 @@ def ensemble_leg(pm, n_systems, steps, graph_steps, seed=0x50524953 + 3):
                                 "p99": round(float(np.percentile(lat, 99)), 4)},
             "plain_step_latency_ms_all_systems_sequential": round(float(sum(plain_lat)) * 1e3, 4),
             "systems_with_faults": bad,
+            "autotune": [{k: e[k] for k in ("lo", "hi", "sort_cell_A", "resort_every", "us_per_step")} for e in table],
             "sizes": "log-uniform 500-20000 atoms", "temperatures_K": "uniform 280-340"}
 
 
@@ -274,9 +367,14 @@ def main():
     ap.add_argument("--ensemble-systems", type=int, default=256, help="batched ensemble size (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=400)
     ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--tune-buckets", type=int, default=4, help="autotuner size buckets for the ensemble (0 = defaults)")
     ap.add_argument("--serving-seconds", type=float, default=60.0, help="serving loop run time (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=1.0, help="job arrivals per second")
     ap.add_argument("--serving-budget-gib", type=float, default=16.0)
+    ap.add_argument("--nve-steps", type=int, default=100000, help="NVE drift leg length (0 = skip)")
+    ap.add_argument("--nve-dt", type=float, default=0.001, help="NVE drift leg timestep (ps)")
+    ap.add_argument("--large-atoms", type=int, default=250000, help="large-system leg size (0 = skip)")
+    ap.add_argument("--large-steps", type=int, default=2000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -337,8 +435,17 @@ def main():
     eng.close()
     out["ensemble"] = None
     if args.ensemble_systems > 0:
-        out["ensemble"] = ensemble_leg(pm, args.ensemble_systems, args.ensemble_steps, args.graph_steps)
+        out["ensemble"] = ensemble_leg(pm, args.ensemble_systems, args.ensemble_steps, args.graph_steps,
+                                       args.tune_buckets)
         log(f"ensemble: {out['ensemble']}")
+    out["large_system"] = None
+    if args.large_atoms > 0:
+        out["large_system"] = large_leg(pm, args.large_atoms, args.large_steps, 200, args.sample_every)
+        log(f"large system: {out['large_system']}")
+    out["nve"] = None
+    if args.nve_steps > 0:
+        out["nve"] = nve_leg(pm, 2000, args.nve_steps, args.nve_dt, max(1, args.nve_steps // 20))
+        log(f"nve: {out['nve']}")
     out["serving"] = None
     if args.serving_seconds > 0:
         out["serving"] = serving_leg(pm, args.serving_seconds, args.serving_rate, args.serving_budget_gib,
@@ -348,6 +455,7 @@ def main():
         try:
             out["cpu_baseline"] = cpu_baseline(pm, top, x_min.astype(np.float64), v0.astype(np.float64), params,
                                                args.cpu_seconds)
+            out["cpu_baseline"]["small_chain_full"] = cpu_chain_c1(pm)
         except Exception as ex:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {ex}")
     print(json.dumps(out), flush=True)

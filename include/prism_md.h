@@ -158,7 +158,9 @@ typedef struct PrismMdParamsV1 {
     double   equilibration_gamma_ps; /* gamma at step 0 of the ramp                   */
     int32_t  resort_every;        /* re-sort atoms into blocks every Nth rebuild (8);
                                    * other rebuilds refresh block boxes and rows only */
-    int32_t  reserved[7];
+    float    sort_cell_A;         /* edge of the Hilbert sort cells that order atoms into
+                                   * 64-atom blocks (2.0; 0 selects 2.0)            */
+    int32_t  reserved[6];
 } PrismMdParamsV1;
 
 typedef struct PrismMdStatsV1 {

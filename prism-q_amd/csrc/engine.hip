@@ -142,7 +142,8 @@ static int validate(const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p, std::s
         }
     }
     if (!(p.dt_ps > 0.0) || !(p.cutoff_A > 0.0) || !(p.skin_A >= 0.0) || p.rebuild_every < 1 || !(p.temperature_K >= 0.0) ||
-        !(p.gamma_ps >= 0.0) || p.constraint_max_iter < 1 || !(p.shake_tol > 0.0) || !(p.rattle_tol > 0.0))
+        !(p.gamma_ps >= 0.0) || p.constraint_max_iter < 1 || !(p.shake_tol > 0.0) || !(p.rattle_tol > 0.0) ||
+        !(p.sort_cell_A == 0.0f || (p.sort_cell_A >= 0.25f && p.sort_cell_A <= 64.0f)))
         return bad("parameters");
     return PRISM_MD_OK;
 }
@@ -569,6 +570,7 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     D.constraint_max_iter = p.constraint_max_iter;
     D.sort_bits = 10;
     D.resort_every = std::max(1, p.resort_every);
+    D.sort_cell = p.sort_cell_A > 0.0f ? p.sort_cell_A : 2.0f;
     D.dt = (float)p.dt_ps;
     D.cutoff = (float)p.cutoff_A;
     D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);
@@ -1339,6 +1341,7 @@ int prism_md_default_params(PrismMdParamsV1 *p) {
     p->equilibration_steps = 10000;  // fused_engine.rs:15370-15378
     p->equilibration_gamma_ps = 1000.0;
     p->resort_every = 8;
+    p->sort_cell_A = 2.0f;
     return PRISM_MD_OK;
 }
 

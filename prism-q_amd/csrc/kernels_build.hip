@@ -16,7 +16,6 @@ namespace {
 
 constexpr int SORT_THREADS = 1024;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr float SORT_CELL = 2.0f;  // Hilbert cell edge, A
 
 // 3-D Hilbert index of cell (x, y, z), `bits` bits per axis, after J. Skilling,
 // "Programming the Hilbert curve", AIP Conf. Proc. 707, 381 (2004): the
@@ -146,6 +145,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
         for (int c = 0; c < 3; ++c) { red[c][wave] = lo[c]; red[3 + c][wave] = hi[c]; }
     __syncthreads();
     int bits = 1;
+    const float inv_cell = 1.0f / D.sort_cell;
     for (int c = 0; c < 3; ++c) {
         lo[c] = 3.0e38f;
         hi[c] = -3.0e38f;
@@ -153,18 +153,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
             lo[c] = fminf(lo[c], red[c][w]);
             hi[c] = fmaxf(hi[c], red[3 + c][w]);
         }
-        const unsigned int cells = (unsigned int)fminf((hi[c] - lo[c]) * (1.0f / SORT_CELL), 1023.0f) + 1u;
+        const unsigned int cells = (unsigned int)fminf((hi[c] - lo[c]) * inv_cell, 1023.0f) + 1u;
         bits = max(bits, 32 - __clz(cells));  // bits to hold cell indices 0..cells-1
     }
     bits = min(bits, D.sort_bits);
     const unsigned int cmax = (1u << bits) - 1u;
 
-    // ---- keys: Hilbert index of 2 A cells relative to the box minimum
+    // ---- keys: Hilbert index of sort cells relative to the box minimum
     for (int i = tid; i < n; i += SORT_THREADS) {
         const float4 p = P.x[a0 + i];
-        const unsigned int cx = min(cmax, (unsigned int)fmaxf(0.0f, (p.x - lo[0]) * (1.0f / SORT_CELL)));
-        const unsigned int cy = min(cmax, (unsigned int)fmaxf(0.0f, (p.y - lo[1]) * (1.0f / SORT_CELL)));
-        const unsigned int cz = min(cmax, (unsigned int)fmaxf(0.0f, (p.z - lo[2]) * (1.0f / SORT_CELL)));
+        const unsigned int cx = min(cmax, (unsigned int)fmaxf(0.0f, (p.x - lo[0]) * inv_cell));
+        const unsigned int cy = min(cmax, (unsigned int)fmaxf(0.0f, (p.y - lo[1]) * inv_cell));
+        const unsigned int cz = min(cmax, (unsigned int)fmaxf(0.0f, (p.z - lo[2]) * inv_cell));
         P.keys_a[a0 + i] = hilbert_key(cx, cy, cz, bits);
         P.vals_a[a0 + i] = i;
     }

@@ -9,6 +9,7 @@
 // (:354-386), compute_angle_force (:388-448), compute_dihedral_force
 // (:524-625), compute_nonbonded_force (:643-717) and
 // ordinary_cutoff_potential_offset_v1 (:631-640).
+#include "bonded.hpp"
 #include "device_common.hpp"
 
 namespace prism {
@@ -180,146 +181,6 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
         }
         if (js == 0) P.unit_f[(size_t)u * SUB + il] = make_float4(fx, fy, fz, ENERGY ? e : 0.0f);
     }
-}
-
-__device__ inline float3 ld3(const float4 *x, int a) {
-    const float4 p = x[a];
-    return make_float3(p.x, p.y, p.z);
-}
-__device__ inline double3 ld3d(const float4 *x, int a) {
-    const float4 p = x[a];
-    return make_double3(p.x, p.y, p.z);
-}
-__device__ inline double3 sub3(double3 a, double3 b) { return make_double3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ inline double dot3(double3 a, double3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ inline double3 cross3(double3 a, double3 b) {
-    return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ inline void put(const DevPtrs &P, int slot, double x, double y, double z) {
-    float *c = P.contrib + (size_t)slot * 3;
-    c[0] = (float)x;
-    c[1] = (float)y;
-    c[2] = (float)z;
-}
-
-// One thread per term: bonds, angles, dihedrals, exceptions in that order.
-// Every term writes the force on each of its atoms into that atom's
-// contribution slot (zeros when the term is masked out or degenerate); the
-// gather adds an atom's slots in slot order, so there are no atomics.  The
-// bonded terms and exceptions evaluate in FP64 from the FP32 state: on CDNA4
-// the FP64 vector rate equals the unpacked FP32 rate, and it removes the
-// cancellation of r - r0 and theta - theta0 near equilibrium that limits an
-// FP32 evaluation to ~1e-5 relative.
-template <bool ENERGY>
-__device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
-    int idx = tid;
-    double energy = 0.0;
-    if (idx < P.n_bonds) {
-        const Bond B = P.bonds[idx];
-        const int *dst = P.dst_bond + 2 * idx;
-        double fm = 0.0;
-        double3 r = make_double3(0, 0, 0);
-        if ((mask & PRISM_MD_TERM_BOND) && B.i >= 0) {
-            r = sub3(ld3d(P.x, B.j), ld3d(P.x, B.i));
-            const double len = sqrt(dot3(r, r));
-            if (len >= 1e-8) {
-                const double dr = len - (double)B.r0;
-                fm = -2.0 * (double)B.k * dr / len;
-                energy = (double)B.k * dr * dr;
-            }
-        }
-        if (B.i >= 0) {
-            put(P, dst[0], -fm * r.x, -fm * r.y, -fm * r.z);
-            put(P, dst[1], fm * r.x, fm * r.y, fm * r.z);
-        }
-    } else if ((idx -= P.n_bonds) < P.n_angles) {
-        const Angle A = P.angles[idx];
-        const int *dst = P.dst_angle + 3 * idx;
-        double fm = 0.0;
-        double3 di = make_double3(0, 0, 0), dk = make_double3(0, 0, 0);
-        if ((mask & PRISM_MD_TERM_ANGLE) && A.i >= 0) {
-            const double3 pj = ld3d(P.x, A.j);
-            const double3 a = sub3(ld3d(P.x, A.i), pj), b = sub3(ld3d(P.x, A.k), pj);
-            const double la = sqrt(dot3(a, a)), lb = sqrt(dot3(b, b));
-            if (la >= 1e-8 && lb >= 1e-8) {
-                double cs = dot3(a, b) / (la * lb);
-                cs = fmax(-1.0, fmin(1.0, cs));
-                const double dth = acos(cs) - (double)A.theta0;
-                double sn = sqrt(1.0 - cs * cs);
-                if (sn < 1e-8) sn = 1e-8;
-                fm = -2.0 * (double)A.kf * dth / sn;
-                const double ia = 1.0 / la, ib = 1.0 / lb;
-                di = make_double3((b.x * ib - cs * a.x * ia) * ia, (b.y * ib - cs * a.y * ia) * ia, (b.z * ib - cs * a.z * ia) * ia);
-                dk = make_double3((a.x * ia - cs * b.x * ib) * ib, (a.y * ia - cs * b.y * ib) * ib, (a.z * ia - cs * b.z * ib) * ib);
-                energy = (double)A.kf * dth * dth;
-            }
-        }
-        if (A.i >= 0) {
-            put(P, dst[0], -fm * di.x, -fm * di.y, -fm * di.z);
-            put(P, dst[1], fm * (di.x + dk.x), fm * (di.y + dk.y), fm * (di.z + dk.z));
-            put(P, dst[2], -fm * dk.x, -fm * dk.y, -fm * dk.z);
-        }
-    } else if ((idx -= P.n_angles) < P.n_dih) {
-        const Dihedral Q = P.dih[idx];
-        const int *dst = P.dst_dih + 4 * idx;
-        double3 f1 = make_double3(0, 0, 0), f4 = make_double3(0, 0, 0), sv = make_double3(0, 0, 0);
-        if ((mask & PRISM_MD_TERM_DIHEDRAL) && Q.i >= 0) {
-            const double3 pj = ld3d(P.x, Q.j), pk = ld3d(P.x, Q.k);
-            const double3 v0 = sub3(ld3d(P.x, Q.i), pj), v1 = sub3(pk, pj), v2 = sub3(pk, ld3d(P.x, Q.l));
-            const double3 c0 = cross3(v0, v1), c1 = cross3(v1, v2);
-            const double n1 = dot3(c0, c0), n2 = dot3(c1, c1), nb = dot3(v1, v1);
-            if (n1 >= 1e-16 && n2 >= 1e-16 && nb >= 1e-16) {
-                const double lb = sqrt(nb);
-                // phi from atan2 of |c0 x c1| and c0.c1: the reference's acos /
-                // asin branches (nhs_amber_fused.cu:562-580) in exact arithmetic
-                const double3 cc = cross3(c0, c1);
-                double phi = atan2(sqrt(dot3(cc, cc)), dot3(c0, c1));
-                if (dot3(v0, c1) < 0.0) phi = -phi;
-                const double nn = (double)Q.n;
-                double sa, ca;
-                sincos(nn * phi - (double)Q.phase, &sa, &ca);
-                const double dE = -(double)Q.kf * nn * sa;
-                const double ff0 = (-dE * lb) / n1, ff3 = (dE * lb) / n2;
-                const double ff1 = dot3(v0, v1) / nb, ff2 = dot3(v2, v1) / nb;
-                f1 = make_double3(ff0 * c0.x, ff0 * c0.y, ff0 * c0.z);
-                f4 = make_double3(ff3 * c1.x, ff3 * c1.y, ff3 * c1.z);
-                sv = make_double3(ff1 * f1.x - ff2 * f4.x, ff1 * f1.y - ff2 * f4.y, ff1 * f1.z - ff2 * f4.z);
-                energy = (double)Q.kf * (1.0 + ca);
-            }
-        }
-        if (Q.i >= 0) {
-            put(P, dst[0], f1.x, f1.y, f1.z);
-            put(P, dst[1], sv.x - f1.x, sv.y - f1.y, sv.z - f1.z);
-            put(P, dst[2], -sv.x - f4.x, -sv.y - f4.y, -sv.z - f4.z);
-            put(P, dst[3], f4.x, f4.y, f4.z);
-        }
-    } else if ((idx -= P.n_dih) < P.n_exc) {
-        const Exception14 X = P.exc[idx];
-        const int *dst = P.dst_exc + 2 * idx;
-        double f = 0.0;
-        double3 d = make_double3(0, 0, 0);
-        if ((mask & PRISM_MD_TERM_EXCEPTION) && X.i >= 0) {
-            d = sub3(ld3d(P.x, X.i), ld3d(P.x, X.j));
-            const double r2 = dot3(d, d);
-            if (r2 < 1e-8) {  // coincident exception atoms (nhs_amber_fused.cu:3338-3343)
-                atomicOr(&P.sst[P.atom_sys[X.i]].status, ST_EXCEPTION_COINCIDENT);
-            } else {
-                const double inv_r2 = 1.0 / r2, inv_r = sqrt(inv_r2);
-                const double s2 = (double)X.sigma * (double)X.sigma * inv_r2, s6 = s2 * s2 * s2;
-                const double qq = COULOMB * (double)X.qq;
-                const double eps = X.eps;
-                f = (24.0 * eps * s6 * (2.0 * s6 - 1.0) + qq * inv_r) * inv_r2;
-                energy = 4.0 * eps * (s6 * s6 - s6) + qq * inv_r;
-            }
-        }
-        if (X.i >= 0) {
-            put(P, dst[0], f * d.x, f * d.y, f * d.z);
-            put(P, dst[1], -f * d.x, -f * d.y, -f * d.z);
-        }
-    } else {
-        return;
-    }
-    if (ENERGY) P.term_e[tid] = energy;
 }
 
 template <bool ENERGY>

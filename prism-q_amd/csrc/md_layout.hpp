@@ -68,7 +68,7 @@ struct SysDesc {
     int warm_tail, equilibration_steps;
     uint32_t seed_lo, stream;
     int noise_explicit;             // 1: read normals from the explicit array
-    int pad1;
+    float sort_cell;                // Hilbert sort cell edge, A
     // term ranges of this system in the global term arrays (energy sums)
     int bond_begin, n_bonds, angle_begin, n_angles;
     int dih_begin, n_dih, exc_begin, n_exc;

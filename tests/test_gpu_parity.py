@@ -49,6 +49,7 @@ def _check_terms(eng, oracle, top_x, x):
         F, E = eng.compute_forces(mask)
         Fo, Eo = oracle.forces(top_x, x.astype(np.float64), mask, True)
         r = rel_rms(F.astype(np.float64), Fo)
+        print(f"[forces] {name:9s} rel RMS {r:.2e}", flush=True)
         assert r <= 1e-5, (name, r)
         if mask != 31:
             k = [m for m, _ in TERMS].index(mask)
@@ -73,6 +74,18 @@ def test_forces_thermal(system, oracle):
     x = eng.get_positions()
     _check_terms(eng, oracle, top.with_positions(x), x)
     assert eng.stats()["status"] == 0
+
+
+def test_newton_third_law(system, pm):
+    """Each pair is evaluated from both of its atoms with the separation
+    negated, which gives exactly opposite pair forces, so the nonbonded forces
+    sum to zero up to the FP32 rounding of the partial sums (relative to the
+    largest atom force)."""
+    top, eng = system
+    F, _ = eng.compute_forces(pm.TERM_NONBONDED)
+    net = np.abs(F.astype(np.float64).sum(axis=0)).max()
+    print(f"[newton] |sum F| {net:.3e}  max|F| {np.abs(F).max():.3e}", flush=True)
+    assert net <= 1e-5 * np.abs(F).max() * np.sqrt(len(F))
 
 
 def test_fixed_point_forces_reproducible(system):
