@@ -75,6 +75,8 @@ __device__ inline void tile_pairs(const float4 xi, const float2 li, int n_iter, 
 // into LDS.  The row is a full list (both directions), so each lane only
 // accumulates its own i-force; the four j-slot partials are summed at the end
 // of the unit and stored as the unit's partial (no atomics).
+constexpr int N_XCD = 8;
+
 template <bool ENERGY>
 __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -84,11 +86,22 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     float4 *jpos = jpos_s[wave];
     float4 *jpar = jpar_s[wave];
     const int n_units = P.counters->n_units;
-    const int n_waves = gridDim.x * NB_WAVES;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    // Static round-robin over the unit list.  (A shared claim counter was
-    // measured far slower: device-scope atomics on one address serialise.)
-    for (int u = blockIdx.x * NB_WAVES + wave; u < n_units; u += n_waves) {
+    // Workgroups are dispatched round-robin over the XCDs (workgroup w on XCD
+    // w % 8, each XCD with its own L2).  Each XCD takes one contiguous eighth
+    // of the unit list -- blocks are space-filling-curve ordered, so that is a
+    // compact region whose j-tiles stay in its L2 -- and its waves go round
+    // robin over it.  (A shared claim counter was measured far slower:
+    // device-scope atomics on one address serialise.)
+    int u0 = blockIdx.x * NB_WAVES + wave, u_end = n_units, stride = gridDim.x * NB_WAVES;
+    if (gridDim.x % N_XCD == 0) {
+        const int xcd = blockIdx.x % N_XCD;
+        const int span_u = (n_units + N_XCD - 1) / N_XCD;
+        u0 = xcd * span_u + (blockIdx.x / N_XCD) * NB_WAVES + wave;
+        u_end = min(n_units, (xcd + 1) * span_u);
+        stride = (gridDim.x / N_XCD) * NB_WAVES;
+    }
+    for (int u = u0; u < u_end; u += stride) {
         const int code = P.units[u];
         const int b = code >> 8, q = (code >> 6) & 3, c = code & 63;
         const int s = P.block_sys[b];
@@ -373,7 +386,9 @@ template <bool ENERGY> static void launch_nb(const DevPtrs &P, hipStream_t st) {
     // persistent grid: every wave resident at once (no second round of
     // workgroups), but no more waves than a typical unit list needs
     const int want = (P.n_blocks_arena * QUARTERS * 12 + NB_WAVES - 1) / NB_WAVES;
-    const int grid = max(1, min(want, resident_blocks<ENERGY>()));
+    const int res = resident_blocks<ENERGY>();
+    int grid = max(1, min(want, res));
+    if (res % N_XCD == 0) grid = (grid + N_XCD - 1) / N_XCD * N_XCD;  // whole XCD rounds
     hipLaunchKernelGGL(k_nonbonded<ENERGY>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
 }
 

@@ -349,6 +349,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
     const int lane = threadIdx.x & 63;
     const int gw = blockIdx.x * IW + (threadIdx.x >> 6);
     __shared__ int last;
+    bool publish = false;  // this thread wrote something the last workgroup reads
     const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
     if (s >= 0 && P.sys[s].active && !P.sst[s].halted) {
         const SysDesc &D = P.sys[s];
@@ -364,7 +365,9 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         int most_ra = 0;
         if (load_group(P, gw * 64 + lane, g, gp)) {
             half_kick(g, P, D.dt, gamma);
-            note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_3, g.a[0], step, most_ra);
+            int sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            publish |= sw < 0;
+            note_sweeps(S, sw, FAULT_RATTLE_3, g.a[0], step, most_ra);
             if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 2, noise);
             FOR_ATOMS(k) {  // nhs_velocity_safety_clamp (nhs_amber_fused.cu:11148-11186)
                 const float thermal = sqrtf((float)KCAL_TO_AMU_A2_PS2 * (float)KB * T / fmaxf(g.m[k], 1.0f));
@@ -375,9 +378,14 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
                     for (int c = 0; c < 3; ++c) g.v[k][c] *= sc;
                 }
             }
-            note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_4, g.a[0], step, most_ra);
+            sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            publish |= sw < 0;
+            note_sweeps(S, sw, FAULT_RATTLE_4, g.a[0], step, most_ra);
             FOR_ATOMS(k) {
-                if (!finite3(g.v[k])) record_fault(S, FAULT_NONFINITE, g.a[k], step);
+                if (!finite3(g.v[k])) {
+                    record_fault(S, FAULT_NONFINITE, g.a[k], step);
+                    publish = true;
+                }
                 P.v[g.a[k]] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
                 for (int c = 0; c < 3; ++c) mv[c] += (double)g.m[k] * g.v[k][c];
             }
@@ -385,13 +393,19 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
         if (next_com) {
             for (int c = 0; c < 3; ++c) mv[c] = wave_sum(mv[c]);
-            if (lane == 0)
+            if (lane == 0) {
                 for (int c = 0; c < 3; ++c) P.com_partial[(size_t)gw * 3 + c] = mv[c];
+                publish = true;
+            }
         }
     }
     // Last-block handshake (no waiting): the workgroup that retires last sums
     // the COM partials in wave order and advances every system's step counter.
-    __threadfence();
+    // A workgroup releases its writes (agent-scope fence) only when the last
+    // one reads any of them -- COM partials, a latched fault: the fence writes
+    // back this XCD's L2 and costs microseconds per workgroup, so on the other
+    // steps the counter increment goes out unfenced.
+    if (__syncthreads_or(publish)) __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(&P.counters->done_blocks, 1u) == gridDim.x - 1;
     __syncthreads();
