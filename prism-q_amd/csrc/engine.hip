@@ -723,7 +723,7 @@ int Engine::refresh(int only) {
                                          hipMemcpyHostToDevice, stream_),
                           "flag"))
         return rc;
-    launch_sort(P_, stream_);
+    launch_sort(P_, false, stream_);
     launch_rows(P_, stream_);
     launch_consume(P_, -1, stream_);
     full_force();
@@ -734,26 +734,28 @@ int Engine::refresh(int only) {
 // bonded, then the fixed-order gather.  (Running the bonded terms on a forked
 // second stream beside the nonbonded kernel was measured slower: the fork and
 // join cost more than the overlap gains.)
-void Engine::launch_forces() {
+void Engine::launch_forces(bool bonded) {
     const bool sample = timing_every_ > 0 && timing_counter_++ % (uint64_t)timing_every_ == 0 &&
                         2 * ev_used_ + 1 < ev_pool_.size();
     if (sample) hipEventRecord(ev_pool_[2 * ev_used_], stream_);
     launch_nonbonded(P_, false, stream_);
     if (sample) hipEventRecord(ev_pool_[2 * ev_used_++ + 1], stream_);
-    launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
+    if (bonded) launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
     launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
 }
 
 void Engine::full_force() {
-    launch_forces();
+    launch_forces(true);
     ++force_evals_;
 }
 
+// The bonded terms ride in the sort kernel's launch (spare workgroups after
+// the sort and refresh ones), which saves the step one kernel.
 void Engine::launch_step(const float *noise_n, const float *noise_np1) {
     launch_first_half(P_, noise_n, stream_);
-    launch_sort(P_, stream_);
+    launch_sort(P_, true, stream_);
     launch_rows(P_, stream_);
-    launch_forces();
+    launch_forces(false);
     launch_second_half(P_, noise_np1, stream_);
     ++force_evals_;
 }
@@ -1104,7 +1106,7 @@ int Engine::minimize(int max_iter, double f_tol, double *e_final, double *f_max)
             alpha = 0.1;
         }
         launch_fire_update(P_, d_vmin_, cv, cf, (float)dt, 0.1f, stream_);
-        launch_sort(P_, stream_);
+        launch_sort(P_, false, stream_);
         launch_rows(P_, stream_);
         launch_consume(P_, -1, stream_);
     }

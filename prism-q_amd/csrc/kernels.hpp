@@ -72,7 +72,10 @@ struct RefGrid {                    // reference cell grid of the pair-list pari
 // then rows and the unit list.  Inside a step the flags are consumed by
 // k_second_half (lists built at x_(s+1) after SHAKE of step s); after a
 // rebuild between steps launch_consume(step_offset = -1) does it (at x_s).
-void launch_sort(const DevPtrs &P, hipStream_t s);
+// with_bonded: the sort's launch also evaluates every bonded term and
+// exception (all terms, no energies) into the contribution slots, as
+// launch_bonded(P, PRISM_MD_TERM_ALL, false) does; the MD step uses it.
+void launch_sort(const DevPtrs &P, bool with_bonded, hipStream_t s);
 void launch_rows(const DevPtrs &P, hipStream_t s);
 void launch_consume(const DevPtrs &P, int step_offset, hipStream_t s);
 // slot_of / xq / lj of system s's n_slots sorted slots from its uploaded perm

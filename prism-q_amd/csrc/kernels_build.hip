@@ -8,6 +8,7 @@
 // per-atom row of <= 1024 neighbour ids at 12 A, rebuilt after every SHAKE; this
 // build keeps block-pair tiles at cutoff + skin and rebuilds when an atom has
 // moved more than skin/2 or after rebuild_every steps.
+#include "bonded.hpp"
 #include "device_common.hpp"
 
 namespace prism {
@@ -98,16 +99,25 @@ __global__ __launch_bounds__(256) void k_restore_slots(DevPtrs P, int s) {
 }
 
 // Workgroups [0, n_sys): one per system, a full re-sort when the system's
-// rebuild is due and resort_due() holds.  Workgroups from n_sys on: one wave
-// per block, a box refresh of blocks whose system rebuilds without re-sorting.
-// Nothing here writes the flags it reads; they are consumed at the end of the
-// step (k_second_half) or by k_consume after a rebuild between steps.
-__global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
+// rebuild is due and resort_due() holds.  Workgroups [n_sys, bonded_wg0): one
+// wave per block, a box refresh of blocks whose system rebuilds without
+// re-sorting.  Workgroups from bonded_wg0 on (only in the MD step's launch):
+// the bonded terms and exceptions of the step's force evaluation, one per
+// thread -- they read the positions the sort leaves untouched and write
+// contribution slots that only the gather reads, and riding here saves the
+// step a kernel launch.  Nothing here writes the flags it reads; they are
+// consumed at the end of the step (k_second_half) or by k_consume after a
+// rebuild between steps.
+__global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int bonded_wg0) {
     __shared__ float red[6][SORT_WAVES];
     __shared__ unsigned int hist[256];
     __shared__ unsigned int running[256];
     __shared__ unsigned int wave_cnt[SORT_WAVES][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if ((int)blockIdx.x >= bonded_wg0) {
+        bonded_term<false>(P, PRISM_MD_TERM_ALL, ((int)blockIdx.x - bonded_wg0) * SORT_THREADS + tid);
+        return;
+    }
     if (P.counters->any_rebuild == 0) return;  // nothing to rebuild this step: one load
     if ((int)blockIdx.x >= P.n_sys) {
         const int b = ((int)blockIdx.x - P.n_sys) * SORT_WAVES + wave;
@@ -170,29 +180,48 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
     }
     __syncthreads();
 
-    // ---- stable LSD radix sort, 8-bit digits, one workgroup
+    // ---- stable LSD radix sort, 8-bit digits, one workgroup.  Each wave owns
+    // one contiguous segment of the keys; a pass counts every wave's digits,
+    // turns the counts into (digit, wave) offsets -- digit-major, wave-minor,
+    // which keeps the pass stable -- and then every wave scatters its segment
+    // in order with its own running offsets (a 64-key chunk at a time, ranks
+    // inside the chunk from ballots).  Four workgroup barriers per pass.
     unsigned int *kin = P.keys_a + a0, *kout = P.keys_b + a0;
     int *vin = P.vals_a + a0, *vout = P.vals_b + a0;
     const int total_bits = 3 * bits;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int seg = (n + SORT_WAVES - 1) / SORT_WAVES;
+    const int s0 = min(n, wave * seg), s1 = min(n, s0 + seg);
     for (int shift = 0; shift < total_bits; shift += 8) {
-        if (tid < 256) hist[tid] = 0;
         for (int k = 0; k < 4; ++k) wave_cnt[wave][lane + 64 * k] = 0;
         __syncthreads();
-        for (int i = tid; i < n; i += SORT_THREADS) atomicAdd(&hist[(kin[i] >> shift) & 255u], 1u);
+        for (int i = s0 + lane; i < s1; i += 64) atomicAdd(&wave_cnt[wave][(kin[i] >> shift) & 255u], 1u);
         __syncthreads();
-        if (tid == 0) {  // exclusive scan (256 entries, serial)
-            unsigned int acc = 0;
-            for (int d = 0; d < 256; ++d) {
-                const unsigned int c = hist[d];
-                running[d] = acc;
+        if (tid < 256) {  // exclusive scan of the digit totals: waves 0..3, 64 digits each
+            unsigned int tot = 0;
+            for (int w = 0; w < SORT_WAVES; ++w) tot += wave_cnt[w][tid];
+            unsigned int inc = tot;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned int y = __shfl_up(inc, o);
+                if (lane >= o) inc += y;
+            }
+            if (lane == 63) hist[wave] = inc;  // wave totals, wave < 4
+            running[tid] = inc - tot;
+        }
+        __syncthreads();
+        if (tid < 256) {
+            unsigned int acc = running[tid];
+            for (int w = 0; w < wave; ++w) acc += hist[w];
+            for (int w = 0; w < SORT_WAVES; ++w) {
+                const unsigned int c = wave_cnt[w][tid];
+                wave_cnt[w][tid] = acc;  // where wave w's keys with digit tid start
                 acc += c;
             }
         }
         __syncthreads();
-        for (int base = 0; base < n; base += SORT_THREADS) {
-            const int i = base + tid;
-            const bool valid = i < n;
+        for (int base = s0; base < s1; base += 64) {
+            const int i = base + lane;
+            const bool valid = i < s1;
             const unsigned int key = valid ? kin[i] : 0u;
             const int val = valid ? vin[i] : 0;
             const unsigned int d = (key >> shift) & 255u;
@@ -204,31 +233,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P) {
                 peers &= bit ? m : ~m;
             }
             const int rank = __popcll(peers & lt);
-            if (valid && rank == 0) wave_cnt[wave][d] = (unsigned int)__popcll(peers);
-            __syncthreads();
-            if (tid < 256) {
-                unsigned int acc = running[tid];
-                for (int w = 0; w < SORT_WAVES; ++w) {
-                    const unsigned int c = wave_cnt[w][tid];
-                    wave_cnt[w][tid] = acc;  // becomes the wave's offset for digit tid
-                    acc += c;
-                }
-                running[tid] = acc;
-            }
-            __syncthreads();
+            const unsigned int first = wave_cnt[wave][d];
             if (valid) {
-                const unsigned int dst = wave_cnt[wave][d] + rank;
-                kout[dst] = key;
-                vout[dst] = val;
+                kout[first + rank] = key;
+                vout[first + rank] = val;
             }
-            __syncthreads();
-            if (tid < 256)
-                for (int w = 0; w < SORT_WAVES; ++w) wave_cnt[w][tid] = 0;
-            __syncthreads();
+            wave_lds_fence();  // every lane has read its digit's offset
+            if (valid && rank == 0) wave_cnt[wave][d] = first + (unsigned int)__popcll(peers);
+            wave_lds_fence();
         }
+        __syncthreads();
         unsigned int *tk = kin; kin = kout; kout = tk;
         int *tv = vin; vin = vout; vout = tv;
-        __syncthreads();
     }
 
     // ---- gather into sorted slots
@@ -409,9 +425,12 @@ __global__ __launch_bounds__(256) void k_consume(DevPtrs P, int step_offset) {
 
 }  // namespace
 
-void launch_sort(const DevPtrs &P, hipStream_t st) {
+void launch_sort(const DevPtrs &P, bool with_bonded, hipStream_t st) {
     const int refresh = (P.n_blocks_arena + SORT_WAVES - 1) / SORT_WAVES;
-    hipLaunchKernelGGL(k_sort, dim3(P.n_sys + refresh), dim3(SORT_THREADS), 0, st, P);
+    const int wg0 = P.n_sys + refresh;
+    const int terms = with_bonded ? P.n_bonds + P.n_angles + P.n_dih + P.n_exc : 0;
+    const int bonded = (terms + SORT_THREADS - 1) / SORT_THREADS;
+    hipLaunchKernelGGL(k_sort, dim3(wg0 + bonded), dim3(SORT_THREADS), 0, st, P, wg0);
 }
 
 void launch_rows(const DevPtrs &P, hipStream_t st) {
