@@ -460,7 +460,9 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
         for (int j : sk) h.skip_idx[L.skip_base + sk_n++] = o + j;
         for (int j : ex) h.ex_idx[L.ex_base + ex_n++] = o + j;
     }
-    // constraint groups: clusters, then lone atoms, then padding
+    // constraint groups: clusters, then lone atoms, then padding (topology
+    // order keeps a wave's atoms close in memory; ordering the clusters by
+    // hydrogen count to give waves one solver shape was measured slower)
     std::vector<char> in_cluster(n, 0);
     int ng = 0;
     auto put_group = [&](const Group &G, const GroupParam &Q) {
@@ -734,28 +736,33 @@ int Engine::refresh(int only) {
 // bonded, then the fixed-order gather.  (Running the bonded terms on a forked
 // second stream beside the nonbonded kernel was measured slower: the fork and
 // join cost more than the overlap gains.)
-void Engine::launch_forces(bool bonded) {
+void Engine::launch_forces(bool step) {
     const bool sample = timing_every_ > 0 && timing_counter_++ % (uint64_t)timing_every_ == 0 &&
                         2 * ev_used_ + 1 < ev_pool_.size();
     if (sample) hipEventRecord(ev_pool_[2 * ev_used_], stream_);
-    launch_nonbonded(P_, false, stream_);
+    if (step) launch_nonbonded_direct(P_, stream_);
+    else launch_nonbonded(P_, false, stream_);
     if (sample) hipEventRecord(ev_pool_[2 * ev_used_++ + 1], stream_);
-    if (bonded) launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
-    launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
+    if (!step) {
+        launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
+        launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);
+    }
 }
 
 void Engine::full_force() {
-    launch_forces(true);
+    launch_forces(false);
     ++force_evals_;
 }
 
-// The bonded terms ride in the sort kernel's launch (spare workgroups after
-// the sort and refresh ones), which saves the step one kernel.
+// Four kernels besides the rebuild pair: the bonded terms ride in the sort
+// kernel's launch (spare workgroups after the sort and refresh ones) and the
+// nonbonded kernel adds every contribution straight into the accumulators
+// the first half zeroed, so no bonded or gather kernel runs.
 void Engine::launch_step(const float *noise_n, const float *noise_np1) {
     launch_first_half(P_, noise_n, stream_);
     launch_sort(P_, true, stream_);
     launch_rows(P_, stream_);
-    launch_forces(false);
+    launch_forces(true);
     launch_second_half(P_, noise_np1, stream_);
     ++force_evals_;
 }
@@ -1298,8 +1305,11 @@ int Engine::checkpoint_restore(int s, const void *buf, size_t len) {
     if ((rc = d2h(&after, P_.sst + s, sizeof after))) return rc;
     if (after.status & (ST_ROW_OVERFLOW | ST_MASK_OVERFLOW)) return fail(PRISM_MD_E_CAPACITY, "restore: tile rows overflow");
     // the saved bookkeeping replaces the rebuild flags; the unit list claimed
-    // above stays (any_dirty is cleared at the end of the next step)
-    if ((rc = hipcheck(hipMemcpyAsync(P_.sst + s, &hd.state, sizeof hd.state, hipMemcpyHostToDevice, stream_), "restore state")))
+    // above stays, so the dirty flag is cleared here -- left set, the next
+    // step's row kernel would claim every block's units a second time
+    const int clean = 0;
+    if ((rc = hipcheck(hipMemcpyAsync(P_.sst + s, &hd.state, sizeof hd.state, hipMemcpyHostToDevice, stream_), "restore state")) ||
+        (rc = hipcheck(hipMemcpyAsync(&P_.counters->any_dirty, &clean, sizeof clean, hipMemcpyHostToDevice, stream_), "restore counters")))
         return rc;
     full_force();  // the carried force F(x)
     if ((rc = hipcheck(hipGetLastError(), "restore"))) return rc;

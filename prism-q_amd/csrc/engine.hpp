@@ -136,7 +136,7 @@ class Engine {
     DevPtrs P_{};
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
-    void launch_forces(bool bonded);  // nonbonded, bonded (unless the step's sort launch carries it), gather
+    void launch_forces(bool step);  // nonbonded, bonded, gather; or the step's direct nonbonded only
     std::vector<void *> allocs_;
     int timing_every_ = 0;        // sampled nonbonded timing (kernel_timing)
     std::vector<hipEvent_t> ev_pool_;  // pairs: [2k] before, [2k+1] after

@@ -67,6 +67,13 @@ __device__ inline void tile_pairs(const float4 xi, const float2 li, int n_iter, 
     }
 }
 
+constexpr int N_XCD = 8;
+
+__device__ inline long long wave_xor_ll(long long v, int m) {
+    const int lo = __shfl_xor((int)(v & 0xffffffffll), m), hi = __shfl_xor((int)(v >> 32), m);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // Persistent grid over the compact unit list.  A unit is (i-block b, quarter
 // q, chunk c of CHUNK tiles of b's row).  Lane l works on i-atom
 // b*64 + q*16 + (l & 15) against j-slot l >> 4: every instruction evaluates
@@ -75,9 +82,16 @@ __device__ inline void tile_pairs(const float4 xi, const float2 li, int n_iter, 
 // into LDS.  The row is a full list (both directions), so each lane only
 // accumulates its own i-force; the four j-slot partials are summed at the end
 // of the unit and stored as the unit's partial (no atomics).
-constexpr int N_XCD = 8;
-
-template <bool ENERGY>
+//
+// DIRECT (the MD step's evaluation): instead of storing the partial for a
+// separate gather, each unit converts it to int64 fixed point and adds it to
+// the atom's force accumulator with an integer atomic; the first unit of a
+// quarter also adds the quarter's bonded contribution slots.  Integer
+// addition is associative, so the force is bitwise the one the gather forms
+// (the same fixed-point terms, summed in another order), and the step loses
+// a kernel and the partials' round trip through HBM.  The accumulators are
+// zeroed by the integrator's first half when it consumes the carried force.
+template <bool ENERGY, bool DIRECT>
 __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int il = lane & (SUB - 1), js = lane >> 4;
@@ -192,18 +206,38 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
             e += __shfl_xor(e, 16);
             e += __shfl_xor(e, 32);
         }
-        if (js == 0) P.unit_f[(size_t)u * SUB + il] = make_float4(fx, fy, fz, ENERGY ? e : 0.0f);
+        if (!DIRECT) {
+            if (js == 0) P.unit_f[(size_t)u * SUB + il] = make_float4(fx, fy, fz, ENERGY ? e : 0.0f);
+        } else {
+            long long add[3] = {to_fixed(fx, FORCE_FIXED_BITS), to_fixed(fy, FORCE_FIXED_BITS),
+                                to_fixed(fz, FORCE_FIXED_BITS)};
+            if (c == 0) {  // the quarter's bonded slots, lane js taking slots js, js+4, ...
+                long long bs[3] = {0, 0, 0};
+                if (ai >= 0) {
+                    for (int j = P.inc_off[ai] + js; j < P.inc_off[ai + 1]; j += 4) {
+                        const float *cb = P.contrib + (size_t)j * 3;
+                        bs[0] += to_fixed(cb[0], FORCE_FIXED_BITS);
+                        bs[1] += to_fixed(cb[1], FORCE_FIXED_BITS);
+                        bs[2] += to_fixed(cb[2], FORCE_FIXED_BITS);
+                    }
+                }
+                for (int k = 0; k < 3; ++k) {
+                    bs[k] += wave_xor_ll(bs[k], 16);
+                    bs[k] += wave_xor_ll(bs[k], 32);
+                    add[k] += bs[k];
+                }
+            }
+            if (ai >= 0 && js < 3) {
+                const long long v = js == 0 ? add[0] : (js == 1 ? add[1] : add[2]);
+                atomicAdd((unsigned long long *)(P.f + (size_t)ai * 4 + js), (unsigned long long)v);
+            }
+        }
     }
 }
 
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
     bonded_term<ENERGY>(P, mask, blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__device__ inline long long wave_xor_ll(long long v, int m) {
-    const int lo = __shfl_xor((int)(v & 0xffffffffll), m), hi = __shfl_xor((int)(v >> 32), m);
-    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
 // One wave per block quarter (16 sorted slots), four lanes per slot: lane
@@ -368,34 +402,36 @@ __global__ __launch_bounds__(256) void k_pairs(DevPtrs P, int mode, RefGrid g, i
 
 // Workgroups of a nonbonded kernel variant that the device holds at once
 // (occupancy x compute units), queried once per device.
-template <bool ENERGY> static int resident_blocks() {
+template <bool ENERGY, bool DIRECT> static int resident_blocks() {
     static int cache[64];
     int dev = 0;
     (void)hipGetDevice(&dev);
     int &c = cache[dev & 63];
     if (c == 0) {
         int per_cu = 0, cus = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbonded<ENERGY>, 64 * NB_WAVES, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbonded<ENERGY, DIRECT>, 64 * NB_WAVES, 0);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         c = max(1, per_cu) * max(1, cus);
     }
     return c;
 }
 
-template <bool ENERGY> static void launch_nb(const DevPtrs &P, hipStream_t st) {
+template <bool ENERGY, bool DIRECT> static void launch_nb(const DevPtrs &P, hipStream_t st) {
     // persistent grid: every wave resident at once (no second round of
     // workgroups), but no more waves than a typical unit list needs
     const int want = (P.n_blocks_arena * QUARTERS * 12 + NB_WAVES - 1) / NB_WAVES;
-    const int res = resident_blocks<ENERGY>();
+    const int res = resident_blocks<ENERGY, DIRECT>();
     int grid = max(1, min(want, res));
     if (res % N_XCD == 0) grid = (grid + N_XCD - 1) / N_XCD * N_XCD;  // whole XCD rounds
-    hipLaunchKernelGGL(k_nonbonded<ENERGY>, dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
+    hipLaunchKernelGGL((k_nonbonded<ENERGY, DIRECT>), dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
 }
 
 void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st) {
-    if (energy) launch_nb<true>(P, st);
-    else launch_nb<false>(P, st);
+    if (energy) launch_nb<true, false>(P, st);
+    else launch_nb<false, false>(P, st);
 }
+
+void launch_nonbonded_direct(const DevPtrs &P, hipStream_t st) { launch_nb<false, true>(P, st); }
 
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t st) {
     const int n = P.n_bonds + P.n_angles + P.n_dih + P.n_exc;

@@ -229,11 +229,15 @@ __device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float g
 // Half kick with the 1000 kcal/mol/A clamp of prism_asc_used_force
 // (prism_asc_decision.cuh:100-104) when gamma != 0, exact when gamma == 0
 // (nhs_amber_fused.cu:1750-1805, :3667-3703).
-__device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, float gamma) {
+// consume: zero the accumulators after reading them -- the step's force
+// evaluation adds into them (first half only).
+__device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, float gamma, bool consume) {
     FOR_ATOMS(k) {
-        const long long *fa = P.f + (size_t)g.a[k] * 4;
+        long long *fa = P.f + (size_t)g.a[k] * 4;
         float f[3];
         for (int c = 0; c < 3; ++c) f[c] = from_fixed(fa[c], FORCE_FIXED_BITS);
+        if (consume)
+            for (int c = 0; c < 3; ++c) fa[c] = 0;
         if (gamma != 0.0f) {
             const float mag = sqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
             const float q = mag > MAX_FORCE ? MAX_FORCE / mag : 1.0f;
@@ -294,7 +298,13 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
     const SysDesc &D = P.sys[s];
     if (!D.active) return;
     SysState &S = P.sst[s];
-    if (S.halted) return;
+    if (S.halted) {  // frozen, but its forces are still re-evaluated into zeroed accumulators
+        const Group G = P.groups[gw * 64 + lane];
+        for (int k = 0; k < 4; ++k)
+            if (G.atom[k] >= 0)
+                for (int c = 0; c < 3; ++c) P.f[(size_t)G.atom[k] * 4 + c] = 0;
+        return;
+    }
     const unsigned long long step = S.step;
     float T, gamma;
     schedule(D, step + 1, T, gamma);
@@ -312,7 +322,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
         }
         if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 1, noise);
         note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_1, g.a[0], step, most_ra);
-        half_kick(g, P, D.dt, gamma);
+        half_kick(g, P, D.dt, gamma, true);
         note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_2, g.a[0], step, most_ra);
         float xs[4][3];
         FOR_ATOMS(k) {
@@ -364,7 +374,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         double mv[3] = {0.0, 0.0, 0.0};
         int most_ra = 0;
         if (load_group(P, gw * 64 + lane, g, gp)) {
-            half_kick(g, P, D.dt, gamma);
+            half_kick(g, P, D.dt, gamma, false);
             int sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
             publish |= sw < 0;
             note_sweeps(S, sw, FAULT_RATTLE_3, g.a[0], step, most_ra);

@@ -76,3 +76,26 @@ def test_foreign_checkpoints_are_refused(pm, system):
         with pytest.raises(pm.PrismMdError):
             c.restore(bytes(bad))
         c.step(5)  # the engine is still usable after refused restores
+
+
+def test_step_force_equals_recomputed_force(pm, system):
+    """The step adds the nonbonded partials and bonded slots straight into
+    the fixed-point accumulators; a separate evaluation (nonbonded, bonded,
+    gather) at the same positions must give the same integers, so replacing
+    the carried force by a recomputed one mid-run leaves the trajectory
+    bitwise unchanged -- also on the first step after a restore."""
+    top, p = system
+    with pm.FusedEngine(top, p) as a, pm.FusedEngine(top, p) as b:
+        a.step(90)
+        b.step(90)
+        fa = a.compute_forces_fixed()  # overwrites a's carried force with the gather's
+        assert np.any(fa != 0)
+        for e in (a, b):
+            e.step(75)
+        assert np.array_equal(a.get_positions(), b.get_positions())
+        assert np.array_equal(a.get_velocities(), b.get_velocities())
+        # energies use the energy variant of the same evaluation
+        a.energies()
+        for e in (a, b):
+            e.step(40)
+        assert np.array_equal(a.get_positions(), b.get_positions())
