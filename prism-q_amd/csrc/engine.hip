@@ -143,7 +143,8 @@ static int validate(const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p, std::s
     }
     if (!(p.dt_ps > 0.0) || !(p.cutoff_A > 0.0) || !(p.skin_A >= 0.0) || p.rebuild_every < 1 || !(p.temperature_K >= 0.0) ||
         !(p.gamma_ps >= 0.0) || p.constraint_max_iter < 1 || !(p.shake_tol > 0.0) || !(p.rattle_tol > 0.0) ||
-        !(p.sort_cell_A == 0.0f || (p.sort_cell_A >= 0.25f && p.sort_cell_A <= 64.0f)))
+        !(p.sort_cell_A == 0.0f || (p.sort_cell_A >= 0.25f && p.sort_cell_A <= 64.0f)) ||
+        !(p.chunk_tiles == 0 || (p.chunk_tiles >= CHUNK_MIN && p.chunk_tiles <= CHUNK_MAX)))
         return bad("parameters");
     return PRISM_MD_OK;
 }
@@ -573,6 +574,10 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     D.sort_bits = 10;
     D.resort_every = std::max(1, p.resort_every);
     D.sort_cell = p.sort_cell_A > 0.0f ? p.sort_cell_A : 2.0f;
+    // tiles per nonbonded unit: short units balance the persistent grid when
+    // units are few, long ones save per-unit start-up when they are many
+    // (measured best: 10 at 25k atoms, 16 at 250k)
+    D.chunk = p.chunk_tiles > 0 ? p.chunk_tiles : (t->n_atoms < 100000 ? 10 : 16);
     D.dt = (float)p.dt_ps;
     D.cutoff = (float)p.cutoff_A;
     D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);

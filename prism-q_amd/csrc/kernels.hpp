@@ -24,7 +24,7 @@ struct DevPtrs {
     int *row_n, *row_j, *row_m;     // [B], [B][MAX_ROW], [B][MAX_ROW]
     int *unit_count;                // [B] nonbonded work units of each block's row
     int *unit_off;                  // [B] first unit of each block in the unit list
-    int *units;                     // [B * UNITS_PER_BLOCK] (block << 8 | quarter << 6 | chunk)
+    int *units;                     // [B * UNITS_PER_BLOCK] (block << 11 | quarter << 9 | first tile)
     float4 *unit_f;                 // [B * UNITS_PER_BLOCK][SUB] per-unit i-forces (x, y, z, energy)
     // bonded contributions: slot per (term, atom position), atom-major so the
     // slots of one atom are contiguous [inc_off[a], inc_off[a+1])

@@ -303,7 +303,7 @@ __device__ int build_row(const DevPtrs &P, int b, bool resort, int wave, int lan
     }
     if (lane == 0) {
         P.row_n[b] = count;
-        P.unit_count[b] = QUARTERS * ((count + CHUNK - 1) / CHUNK);
+        P.unit_count[b] = QUARTERS * ((count + D.chunk - 1) / D.chunk);
     }
     int *pb = flag_lds[wave];  // the block's partner blocks, ascending
     int n_pb = 0;
@@ -394,7 +394,8 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
     int cnt;
     if (P.sst[s].rebuild_flag) {
         const int tiles = build_row(P, b, resort_due(P.sys[s], P.sst[s]), wave, lane, row_lds, flag_lds, mask_lds);
-        cnt = QUARTERS * ((tiles + CHUNK - 1) / CHUNK);
+        const int ch = P.sys[s].chunk;
+        cnt = QUARTERS * ((tiles + ch - 1) / ch);
     } else {
         cnt = P.unit_count[b];
     }
@@ -403,7 +404,8 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
     off = __shfl(off, 0);
     if (lane == 0) P.unit_off[b] = off;
     const int per_q = cnt / QUARTERS;  // chunks per quarter
-    for (int k = lane; k < cnt; k += 64) P.units[off + k] = (b << 8) | ((k / per_q) << 6) | (k % per_q);
+    const int ch = P.sys[s].chunk;  // the unit's first tile is stored, so decoding needs no descriptor
+    for (int k = lane; k < cnt; k += 64) P.units[off + k] = (b << 11) | ((k / per_q) << 9) | ((k % per_q) * ch);
 }
 
 // Consume the rebuild flags after a rebuild between steps (the integrator

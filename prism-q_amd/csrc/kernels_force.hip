@@ -75,7 +75,7 @@ __device__ inline long long wave_xor_ll(long long v, int m) {
 }
 
 // Persistent grid over the compact unit list.  A unit is (i-block b, quarter
-// q, chunk c of CHUNK tiles of b's row).  Lane l works on i-atom
+// q, chunk c of D.chunk tiles of b's row).  Lane l works on i-atom
 // b*64 + q*16 + (l & 15) against j-slot l >> 4: every instruction evaluates
 // 16 i-atoms x 4 j-atoms.  Per tile the 64 j-atoms are culled against the
 // quarter's current bounding box at the cutoff and the survivors compacted
@@ -115,13 +115,22 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
         u_end = min(n_units, (xcd + 1) * span_u);
         stride = (gridDim.x / N_XCD) * NB_WAVES;
     }
+    // the next unit's code is loaded one unit ahead
+    int code_next = u0 < u_end ? P.units[u0] : 0;
     for (int u = u0; u < u_end; u += stride) {
-        const int code = P.units[u];
-        const int b = code >> 8, q = (code >> 6) & 3, c = code & 63;
+        const int code = code_next;
+        if (u + stride < u_end) code_next = P.units[u + stride];
+        const int b = code >> 11, q = (code >> 9) & 3, t0 = code & (MAX_ROW - 1);
+        // tile indices of up to CHUNK_MAX tiles, loaded before the row length
+        // and the descriptor arrive (entries past the row are not used)
+        const int ti = min(t0 + (lane & (CHUNK_MAX - 1)), MAX_ROW - 1);
+        const int my_jb = P.row_j[(size_t)b * MAX_ROW + ti];
+        const int my_ms = P.row_m[(size_t)b * MAX_ROW + ti];
         const int s = P.block_sys[b];
         const SysDesc &D = P.sys[s];
         const int nrow = P.row_n[b];
-        const int t0 = c * CHUNK, t1 = min(nrow, t0 + CHUNK);
+        const int t1 = min(nrow, t0 + D.chunk);
+        const int c = t0 == 0 ? 0 : 1;  // first unit of the quarter or not
         const float reach = D.cutoff + 0.001f;
         const float reach2 = reach * reach;
         const unsigned lo_bits = __float_as_uint(0.01f);
@@ -149,11 +158,6 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
         // the unit's tile indices and mask slots, one per lane, then a software
         // pipeline: the next tile's j-atoms load while this tile computes
         const int n_t = t1 - t0;
-        int my_jb = 0, my_ms = -1;
-        if (lane < n_t) {
-            my_jb = P.row_j[(size_t)b * MAX_ROW + t0 + lane];
-            my_ms = P.row_m[(size_t)b * MAX_ROW + t0 + lane];
-        }
         int jb = __shfl(my_jb, 0);
         float4 xj = P.xq[jb * BLOCK + lane];
         float2 lj = P.lj[jb * BLOCK + lane];

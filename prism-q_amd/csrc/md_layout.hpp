@@ -22,8 +22,9 @@ constexpr int MAX_ROW = 512;         // j-blocks per i-block row (full list)
 constexpr int MAX_MASKS = 64;        // exclusion-mask tiles per row
 constexpr int SUB = 16;              // i-atoms per nonbonded wave (block quarter)
 constexpr int QUARTERS = BLOCK / SUB;
-constexpr int CHUNK = 8;             // tiles per nonbonded work unit
-constexpr int CHUNKS_PER_ROW = MAX_ROW / CHUNK;
+constexpr int CHUNK_MIN = 8;         // tiles per nonbonded work unit: per system, in
+constexpr int CHUNK_MAX = 32;        // [CHUNK_MIN, CHUNK_MAX] (SysDesc::chunk)
+constexpr int CHUNKS_PER_ROW = MAX_ROW / CHUNK_MIN;
 constexpr int UNITS_PER_BLOCK = QUARTERS * CHUNKS_PER_ROW;
 constexpr int FORCE_FIXED_BITS = 24; // int64 fixed point: 2^24 per kcal/mol/A
 constexpr int ENERGY_FIXED_BITS = 24;
@@ -59,7 +60,8 @@ struct SysDesc {
     float dt;
     float cutoff, cutoff2, list_cutoff;
     float half_skin2;               // (skin/2)^2 displacement trigger
-    float shake_tol_f, rattle_tol_f; // unused (FP64 copies below)
+    int chunk;                      // row tiles per nonbonded work unit (CHUNK_MIN..CHUNK_MAX)
+    int pad_desc;
     double shake_tol, rattle_tol;
     double inv_total_mass;
     // schedule (protocol_director.cu:58-123)

@@ -67,7 +67,8 @@ class ParamsV1(C.Structure):
         ("end_temperature_K", C.c_double), ("cold_hold_end", C.c_int32), ("ramp_end", C.c_int32),
         ("warm_hold_end", C.c_int32), ("ramp_down_end", C.c_int32), ("warm_tail", C.c_int32),
         ("equilibration_steps", C.c_int32), ("equilibration_gamma_ps", C.c_double),
-        ("resort_every", C.c_int32), ("sort_cell_A", C.c_float), ("reserved", C.c_int32 * 6),
+        ("resort_every", C.c_int32), ("sort_cell_A", C.c_float), ("chunk_tiles", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

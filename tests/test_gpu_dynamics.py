@@ -64,14 +64,18 @@ def test_nve_energy_conservation(pm, nve_start):
     """gamma = 0 selects exact kicks (no force clamp).  At 0.5 fs the total
     energy of the relaxed 2,000-atom chain stays within 1 % of the kinetic
     energy over 5 ps, and the error is the integrator's: doubling dt raises
-    it about fourfold (second order), which a force error would not do."""
+    it about fourfold (second order), which a force error would not do.  The
+    ratio is taken of the mean |error| over the ten samples: the last sample
+    alone scatters by tens of percent between runs that differ only in FP32
+    summation order."""
     top, x, v = nve_start
     _, fine, ke = _nve_series(pm, top, x, v, 0.0005, 10, 0.5)
     _, coarse, _ = _nve_series(pm, top, x, v, 0.001, 10, 0.5)
     print(f"[nve] 0.5 fs: {np.round(fine, 2).tolist()}  1 fs: {np.round(coarse, 2).tolist()}  KE {ke:.0f}",
           flush=True)
     assert np.abs(fine).max() < 0.01 * ke
-    assert 2.5 * abs(fine[-1]) < abs(coarse[-1]) < 6.0 * abs(fine[-1])
+    ratio = np.abs(coarse).mean() / np.abs(fine).mean()
+    assert 2.5 < ratio < 6.0, ratio
 
 
 def test_nve_drift_matches_oracle(pm, oracle, nve_start):
