@@ -35,20 +35,18 @@ struct GroupState {
 
 // Solve the k x k (k <= 3) system M lam = b of one cluster, M symmetric
 // positive definite, by cofactors in FP64.  Returns false when M is singular.
-__device__ inline bool solve3(int k, const double M[3][3], const double b[3], double lam[3]) {
-    if (k == 1) {
-        if (!(M[0][0] > 0.0)) return false;
-        lam[0] = b[0] / M[0][0];
-        return true;
-    }
-    if (k == 2) {
-        const double det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
-        if (!(fabs(det) > 1e-300)) return false;
-        const double id = 1.0 / det;
-        lam[0] = (b[0] * M[1][1] - M[0][1] * b[1]) * id;
-        lam[1] = (M[0][0] * b[1] - b[0] * M[1][0]) * id;
-        return true;
-    }
+// It is solved as 3 x 3 with the unused rows and columns set to the identity
+// and their right-hand sides to zero: one code path and one FP64 division for
+// every group shape, so a wave that mixes shapes does not run three solvers
+// one after another.  M and b are overwritten.
+__device__ inline bool solve3(int k, double M[3][3], double b[3], double lam[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (i >= k) {
+            for (int j = 0; j < 3; ++j) M[i][j] = M[j][i] = 0.0;
+            M[i][i] = 1.0;
+            b[i] = 0.0;
+        }
     const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
     const double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
     const double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
@@ -105,21 +103,23 @@ __device__ int rattle(GroupState &g, const GroupParam &gp, double tol, int max_i
             for (int c = 0; c < 3; ++c) dv[c] += lam[j] * r[j][c];
     double vc[3];
     for (int c = 0; c < 3; ++c) vc[c] = (double)g.v[0][c] + imc * dv[c];
-    double worst = 0.0;
-    bool bad = false;
+    // residual test |r.u| / (|r| max(|u|, 1)) <= tol, squared so that the
+    // lane's dependent chain holds no FP64 division or square root
+    const double t = fmax(tol, 1e-12), t2 = t * t;
+    bool bad = false, ok = true;
     FOR_H(h) {
         double vh[3];
         for (int c = 0; c < 3; ++c) vh[c] = (double)g.v[h][c] - imh * lam[h - 1] * r[h - 1][c];
         const double ux = vh[0] - vc[0], uy = vh[1] - vc[1], uz = vh[2] - vc[2];
         const double r2 = r[h - 1][0] * r[h - 1][0] + r[h - 1][1] * r[h - 1][1] + r[h - 1][2] * r[h - 1][2];
         const double res = r[h - 1][0] * ux + r[h - 1][1] * uy + r[h - 1][2] * uz;
-        const double rel = fabs(res) / (sqrt(r2) * fmax(sqrt(ux * ux + uy * uy + uz * uz), 1.0));
-        bad |= !(r2 >= 1e-12) || !isfinite(rel);
-        worst = fmax(worst, rel);
+        const double lhs = res * res, rhs = t2 * r2 * fmax(ux * ux + uy * uy + uz * uz, 1.0);
+        bad |= !(r2 >= 1e-12) || !isfinite(lhs) || !isfinite(rhs);
+        ok &= lhs <= rhs;
         for (int c = 0; c < 3; ++c) g.v[h][c] = (float)vh[c];
     }
     for (int c = 0; c < 3; ++c) g.v[0][c] = (float)vc[c];
-    return (bad || !(worst <= fmax(tol, 1e-12))) ? -1 : 1;
+    return (bad || !ok) ? -1 : 1;
 }
 
 // SHAKE of one cluster (shake_cluster_positions, nhs_amber_fused.cu:1889-2173).
@@ -134,13 +134,14 @@ __device__ int shake(GroupState &g, const float xs[4][3], const GroupParam &gp, 
     if (g.n < 2) return 0;
     const int k = g.n - 1;
     const double imc = gp.inv_mass_c, imh = gp.inv_mass_h;
-    double o[3][3] = {}, r0[3][3] = {}, lam[3] = {0.0, 0.0, 0.0}, d2[3] = {};
+    double o[3][3] = {}, r0[3][3] = {}, lam[3] = {0.0, 0.0, 0.0}, d2[3] = {}, o2[3] = {};
     FOR_H(h) {
         for (int c = 0; c < 3; ++c) {
             o[h - 1][c] = (double)xs[h][c] - (double)xs[0][c];
             r0[h - 1][c] = (double)g.x[h][c] - (double)g.x[0][c];
         }
         d2[h - 1] = (double)gp.d[h - 1] * (double)gp.d[h - 1];
+        o2[h - 1] = o[h - 1][0] * o[h - 1][0] + o[h - 1][1] * o[h - 1][1] + o[h - 1][2] * o[h - 1][2];
     }
     int used = 0;
     bool converged = false, bad = false;
@@ -152,19 +153,19 @@ __device__ int shake(GroupState &g, const float xs[4][3], const GroupParam &gp, 
         for (int j = 0; j < 3; ++j)
             if (j < k)
                 for (int c = 0; c < 3; ++c) sc[c] += lam[j] * o[j][c];
-        double r[3][3] = {}, gres[3] = {}, M[3][3] = {}, dl[3] = {};
-        double worst = 0.0;
+        double r[3][3] = {}, gres[3] = {}, M[3][3] = {}, dl[3] = {}, rr[3] = {};
+        bool within = true;  // every |r^2 - d^2| / d^2 <= tol, tested as |r^2 - d^2| <= tol d^2
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             if (i < k) {
                 for (int c = 0; c < 3; ++c) r[i][c] = r0[i][c] - imh * lam[i] * o[i][c] - imc * sc[c];
-                const double r2 = r[i][0] * r[i][0] + r[i][1] * r[i][1] + r[i][2] * r[i][2];
-                gres[i] = r2 - d2[i];
-                worst = fmax(worst, fabs(gres[i]) / d2[i]);
-                bad |= !isfinite(r2) || !(d2[i] > 0.0);
+                rr[i] = r[i][0] * r[i][0] + r[i][1] * r[i][1] + r[i][2] * r[i][2];
+                gres[i] = rr[i] - d2[i];
+                within &= fabs(gres[i]) <= tol * d2[i];
+                bad |= !isfinite(rr[i]) || !(d2[i] > 0.0);
             }
         if (bad) break;
-        if (worst <= tol) {
+        if (within) {
             converged = true;
             break;
         }
@@ -172,10 +173,10 @@ __device__ int shake(GroupState &g, const float xs[4][3], const GroupParam &gp, 
         for (int i = 0; i < 3; ++i)
             if (i < k) {
                 const double ro = r[i][0] * o[i][0] + r[i][1] * o[i][1] + r[i][2] * o[i][2];
-                // the reference's guard against a bond that rotated ~90 degrees
-                const double scale = sqrt((r[i][0] * r[i][0] + r[i][1] * r[i][1] + r[i][2] * r[i][2]) *
-                                          (o[i][0] * o[i][0] + o[i][1] * o[i][1] + o[i][2] * o[i][2]));
-                bad |= !(scale > 0.0) || fabs(ro) / scale <= 1e-6;
+                // the reference's guard against a bond that rotated ~90 degrees,
+                // |r.o| <= 1e-6 |r| |o|, squared (no square root)
+                const double ss = rr[i] * o2[i];
+                bad |= !(ss > 0.0) || ro * ro <= 1e-12 * ss;
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
                     if (j < k) {
