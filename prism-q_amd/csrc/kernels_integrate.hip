@@ -222,7 +222,10 @@ __device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float g
         } else {
             langevin_normals(D.seed_lo, D.stream, (uint32_t)(g.a[k] - D.atom_begin), step, purpose, xi);
         }
-        const float c2 = sqrtf((1.0f - c1 * c1) * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T / g.m[k]);
+        // hardware reciprocal and square root (~1 ulp): the IEEE-rounded
+        // forms cost ~10 dependent instructions each on a latency-bound lane
+        const float c2 = __builtin_amdgcn_sqrtf((1.0f - c1 * c1) * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T *
+                                                __builtin_amdgcn_rcpf(g.m[k]));
         for (int c = 0; c < 3; ++c) g.v[k][c] = c1 * g.v[k][c] + c2 * xi[c];
     }
 }
@@ -239,12 +242,14 @@ __device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, floa
         for (int c = 0; c < 3; ++c) f[c] = from_fixed(fa[c], FORCE_FIXED_BITS);
         if (consume)
             for (int c = 0; c < 3; ++c) fa[c] = 0;
-        if (gamma != 0.0f) {
-            const float mag = sqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-            const float q = mag > MAX_FORCE ? MAX_FORCE / mag : 1.0f;
-            for (int c = 0; c < 3; ++c) f[c] *= q;
+        if (gamma != 0.0f) {  // |f| > MAX_FORCE tested squared; the rescale is rare
+            const float mag2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+            if (mag2 > MAX_FORCE * MAX_FORCE) {
+                const float q = MAX_FORCE * __builtin_amdgcn_rsqf(mag2);
+                for (int c = 0; c < 3; ++c) f[c] *= q;
+            }
         }
-        const float kick = 0.5f * dt * (float)KCAL_TO_AMU_A2_PS2 / g.m[k];
+        const float kick = 0.5f * dt * (float)KCAL_TO_AMU_A2_PS2 * __builtin_amdgcn_rcpf(g.m[k]);
         for (int c = 0; c < 3; ++c) g.v[k][c] += kick * f[c];
     }
 }
@@ -381,11 +386,12 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
             note_sweeps(S, sw, FAULT_RATTLE_3, g.a[0], step, most_ra);
             if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 2, noise);
             FOR_ATOMS(k) {  // nhs_velocity_safety_clamp (nhs_amber_fused.cu:11148-11186)
-                const float thermal = sqrtf((float)KCAL_TO_AMU_A2_PS2 * (float)KB * T / fmaxf(g.m[k], 1.0f));
-                const float vmax = fmaxf(20.0f * thermal, 2.0f);
-                const float sp = sqrtf(g.v[k][0] * g.v[k][0] + g.v[k][1] * g.v[k][1] + g.v[k][2] * g.v[k][2]);
-                if (sp > vmax) {
-                    const float sc = vmax / sp;
+                // |v| > max(20 sqrt(418.4 kB T / max(m, 1)), 2) tested squared
+                const float vmax2 = fmaxf(400.0f * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T *
+                                              __builtin_amdgcn_rcpf(fmaxf(g.m[k], 1.0f)), 4.0f);
+                const float sp2 = g.v[k][0] * g.v[k][0] + g.v[k][1] * g.v[k][1] + g.v[k][2] * g.v[k][2];
+                if (sp2 > vmax2) {
+                    const float sc = __builtin_amdgcn_sqrtf(vmax2) * __builtin_amdgcn_rsqf(sp2);
                     for (int c = 0; c < 3; ++c) g.v[k][c] *= sc;
                 }
             }

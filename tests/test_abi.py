@@ -90,6 +90,9 @@ def test_invalid_inputs_rejected_before_device(pm, small_top):
     bad = pm.default_params(dt_ps=-1.0)
     rc, h, msg = _create(pm, small_top, bad)
     assert rc == pm.E_INVALID and not h.value and "param" in msg
+    for chunk in (4, 33, -1):  # row tiles per nonbonded unit: 0 (default) or 8..32
+        rc, h, msg = _create(pm, small_top, pm.default_params(chunk_tiles=chunk))
+        assert rc == pm.E_INVALID and not h.value and "param" in msg, chunk
     t = small_top.with_positions(small_top["positions"].copy())
     t.a["positions"][5] = float("nan")
     rc, h, msg = _create(pm, t, pm.default_params())

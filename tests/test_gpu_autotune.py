@@ -1,7 +1,8 @@
 """The autotuner's parameters change only the tile layout, never the
-physics: forces with any sort-cell size and re-sort interval agree with the
-defaults to FP32 rounding and with the FP64 oracle; the tuner runs on the
-device and returns a measured choice for each bucket."""
+physics: forces with any sort-cell size, re-sort interval and tiles per
+nonbonded unit agree with the defaults to FP32 rounding and with the FP64
+oracle; the tuner runs on the device and returns a measured choice for each
+bucket."""
 from __future__ import annotations
 
 import numpy as np
@@ -15,12 +16,15 @@ def test_layout_parameters_do_not_change_forces(pm, oracle):
     x = top["positions"].reshape(-1, 3)
     Fo, _ = oracle.forces(top, x.astype(np.float64))
     base = None
-    for cell, resort in [(2.0, 8), (1.5, 4), (3.0, 16)]:
-        with pm.FusedEngine(top, pm.default_params(sort_cell_A=cell, resort_every=resort)) as eng:
+    for cell, resort, chunk in [(2.0, 8, 0), (1.5, 4, 8), (3.0, 16, 32), (2.0, 8, 13)]:
+        p = pm.default_params(sort_cell_A=cell, resort_every=resort, chunk_tiles=chunk)
+        with pm.FusedEngine(top, p) as eng:
             F, E = eng.compute_forces()
             assert np.array_equal(eng.pairs(1), oracle.pairs(top, x, x, 1))
+            eng.step(30)  # the step's direct accumulation with this layout
+            assert eng.stats()["status"] == 0
         r = float(np.sqrt(((F - Fo) ** 2).sum() / (Fo ** 2).sum()))
-        print(f"[layout] cell {cell} resort {resort}: rel RMS vs oracle {r:.2e}", flush=True)
+        print(f"[layout] cell {cell} resort {resort} chunk {chunk}: rel RMS vs oracle {r:.2e}", flush=True)
         assert r <= 1e-5
         if base is None:
             base = F
