@@ -6,7 +6,8 @@ Workload: the 25,000-atom synthetic AMBER-typed chain (generator seed
 with FP64 SHAKE/RATTLE on X-H bonds, 10 A potential-shift cutoff, 2 A skin,
 pair-list rebuild every 20 steps or on skin/2 displacement, the reference's
 default friction (10/ps after the equilibration ramp from 1000/ps).  The system
-is FIRE-minimised, then W warm-up steps run untimed, then K steps are timed.
+is FIRE-minimised, then W warm-up steps run untimed, then K steps are timed
+(K = 100,000 by default, the workload's full run length).
 
 One JSON line on stdout:
   value        ns/day = steps/s x 0.3456 (4 fs), inputs resident on the device;
@@ -26,10 +27,15 @@ One JSON line on stdout:
   serving      continuous batching of Poisson-arriving jobs admitted into a
                slot pool sized to an HBM budget: p50/p99 per-step latency,
                aggregate atom-steps/s, jobs done, admission wait;
-  large_system the same generator at 250,000 atoms in one globule: ns/day and
-               the nonbonded FP32 rate at that size;
+  large_system the same generator at 250,000 atoms in one globule, 20,000
+               steps (its full run length): ns/day and the nonbonded FP32
+               rate at that size;
   nve          energy drift of the 2,000-atom chain over 100,000 steps without
-               a thermostat, at 1 fs and 0.5 fs (the error is second order in dt);
+               a thermostat, at 1 fs and 0.5 fs.
+
+The ensemble (400 steps per system instead of 10,000) and serving (60 s
+instead of 30 min) legs are shortened so the default run stays within
+minutes; --ensemble-steps and --serving-seconds run them longer.
   cpu_baseline.small_chain_full  the 2,000-atom chain, 1,000 steps at 2 fs,
                through the FP64 oracle in full.
 """
@@ -364,7 +370,7 @@ def traffic_per_launch():
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=100000)
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--atoms", type=int, default=25000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
@@ -382,7 +388,7 @@ def main():
     ap.add_argument("--nve-steps", type=int, default=100000, help="NVE drift leg length (0 = skip)")
     ap.add_argument("--nve-dt", type=float, default=0.001, help="NVE drift leg timestep (ps)")
     ap.add_argument("--large-atoms", type=int, default=250000, help="large-system leg size (0 = skip)")
-    ap.add_argument("--large-steps", type=int, default=2000)
+    ap.add_argument("--large-steps", type=int, default=20000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
