@@ -21,6 +21,15 @@ __device__ inline double dot3(double3 a, double3 b) { return a.x * b.x + a.y * b
 __device__ inline double3 cross3(double3 a, double3 b) {
     return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+// Slot of a term's position q in the arena's contribution layout (TM:
+// term-major, the layout is a kernel template parameter).
+template <bool TM>
+struct Slots {
+    const int *atom_major;  // dst_* row of the term
+    int term_first;         // the term's first slot in the term-major layout
+    __device__ int operator[](int q) const { return TM ? term_first + q : atom_major[q]; }
+};
+
 __device__ inline void put(const DevPtrs &P, int slot, double x, double y, double z) {
     float *c = P.contrib + (size_t)slot * 3;
     c[0] = (float)x;
@@ -29,20 +38,20 @@ __device__ inline void put(const DevPtrs &P, int slot, double x, double y, doubl
 }
 
 // One thread per term: bonds, angles, dihedrals, exceptions in that order.
-// Every term writes the force on each of its atoms into that atom's
+// Every term writes the force on each of its atoms into that position's
 // contribution slot (zeros when the term is masked out or degenerate); the
-// gather adds an atom's slots in slot order, so there are no atomics.  The
+// gather adds an atom's slots as int64 fixed point, so there are no atomics.  The
 // bonded terms and exceptions evaluate in FP64 from the FP32 state: on CDNA4
 // the FP64 vector rate equals the unpacked FP32 rate, and it removes the
 // cancellation of r - r0 and theta - theta0 near equilibrium that limits an
 // FP32 evaluation to ~1e-5 relative.
-template <bool ENERGY>
+template <bool ENERGY, bool TM>
 __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
     int idx = tid;
     double energy = 0.0;
     if (idx < P.n_bonds) {
         const Bond B = P.bonds[idx];
-        const int *dst = P.dst_bond + 2 * idx;
+        const Slots<TM> dst{P.dst_bond + 2 * idx, 2 * idx};
         double fm = 0.0;
         double3 r = make_double3(0, 0, 0);
         if ((mask & PRISM_MD_TERM_BOND) && B.i >= 0) {
@@ -60,7 +69,7 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
         }
     } else if ((idx -= P.n_bonds) < P.n_angles) {
         const Angle A = P.angles[idx];
-        const int *dst = P.dst_angle + 3 * idx;
+        const Slots<TM> dst{P.dst_angle + 3 * idx, 2 * P.n_bonds + 3 * idx};
         double fm = 0.0;
         double3 di = make_double3(0, 0, 0), dk = make_double3(0, 0, 0);
         if ((mask & PRISM_MD_TERM_ANGLE) && A.i >= 0) {
@@ -87,7 +96,7 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
         }
     } else if ((idx -= P.n_angles) < P.n_dih) {
         const Dihedral Q = P.dih[idx];
-        const int *dst = P.dst_dih + 4 * idx;
+        const Slots<TM> dst{P.dst_dih + 4 * idx, 2 * P.n_bonds + 3 * P.n_angles + 4 * idx};
         double3 f1 = make_double3(0, 0, 0), f4 = make_double3(0, 0, 0), sv = make_double3(0, 0, 0);
         if ((mask & PRISM_MD_TERM_DIHEDRAL) && Q.i >= 0) {
             const double3 pj = ld3d(P.x, Q.j), pk = ld3d(P.x, Q.k);
@@ -121,7 +130,7 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
         }
     } else if ((idx -= P.n_dih) < P.n_exc) {
         const Exception14 X = P.exc[idx];
-        const int *dst = P.dst_exc + 2 * idx;
+        const Slots<TM> dst{P.dst_exc + 2 * idx, 2 * P.n_bonds + 3 * P.n_angles + 4 * P.n_dih + 2 * idx};
         double f = 0.0;
         double3 d = make_double3(0, 0, 0);
         if ((mask & PRISM_MD_TERM_EXCEPTION) && X.i >= 0) {

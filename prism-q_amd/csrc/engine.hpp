@@ -120,12 +120,12 @@ class Engine {
         std::vector<Angle> angles;
         std::vector<Dihedral> dih;
         std::vector<Exception14> exc;
-        std::vector<int> dst_b, dst_a, dst_d, dst_e;
+        std::vector<int> dst_b, dst_a, dst_d, dst_e, inc_idx;
     } h_;
     float *d_mass_ = nullptr, *d_q_ = nullptr, *d_sig_ = nullptr, *d_eps_ = nullptr;
     int *d_atom_sys_ = nullptr, *d_block_sys_ = nullptr, *d_wave_sys_ = nullptr;
     int *d_skip_off_ = nullptr, *d_skip_idx_ = nullptr, *d_ex_off_ = nullptr, *d_ex_idx_ = nullptr, *d_inc_off_ = nullptr;
-    int *d_dst_b_ = nullptr, *d_dst_a_ = nullptr, *d_dst_d_ = nullptr, *d_dst_e_ = nullptr;
+    int *d_dst_b_ = nullptr, *d_dst_a_ = nullptr, *d_dst_d_ = nullptr, *d_dst_e_ = nullptr, *d_inc_idx_ = nullptr;
     Group *d_groups_ = nullptr;
     GroupParam *d_gp_ = nullptr;
     Bond *d_bonds_ = nullptr;

@@ -26,11 +26,22 @@ struct DevPtrs {
     int *unit_off;                  // [B] first unit of each block in the unit list
     int *units;                     // [B * UNITS_PER_BLOCK] (block << 11 | quarter << 9 | first tile)
     float4 *unit_f;                 // [B * UNITS_PER_BLOCK][SUB] per-unit i-forces (x, y, z, energy)
-    // bonded contributions: slot per (term, atom position), atom-major so the
-    // slots of one atom are contiguous [inc_off[a], inc_off[a+1])
+    // bonded contributions: one slot per (term, atom position), 3 FP32 each,
+    // in one of two layouts (term_major):
+    //   0  atom-major: the slots of atom a are [inc_off[a], inc_off[a+1]) and
+    //      dst_* give each term position's slot -- the gather reads contiguous
+    //      slots, the terms' stores scatter;
+    //   1  term-major: bonds, angles, dihedrals, exceptions, each term's
+    //      positions adjacent, so a wave of terms stores one contiguous range;
+    //      inc_idx[inc_off[a] .. inc_off[a+1]) lists atom a's slots.
+    // Large arenas use term-major (their bonded stores are bandwidth-bound),
+    // small ones atom-major (their gather is latency-bound).  The forces are
+    // the same integers either way.
     float *contrib;                 // [n_slots][3]
     const int *inc_off;             // [A + 1]
-    const int *dst_bond, *dst_angle, *dst_dih, *dst_exc;  // first slot per term position: [n][arity]
+    const int *dst_bond, *dst_angle, *dst_dih, *dst_exc;  // atom-major slot per term position: [n][arity]
+    const int *inc_idx;             // term-major: [n_slots] slots of each atom
+    int term_major;
     double *term_e;                 // per-term energy (bonds, angles, dihedrals, exceptions)
     double *atom_e;                 // per-atom nonbonded energy of the last gather
     unsigned long long *masks;      // [B][MAX_MASKS][64] per partner block: skip bits of each i-lane

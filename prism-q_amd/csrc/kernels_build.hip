@@ -108,6 +108,7 @@ __global__ __launch_bounds__(256) void k_restore_slots(DevPtrs P, int s) {
 // step a kernel launch.  Nothing here writes the flags it reads; they are
 // consumed at the end of the step (k_second_half) or by k_consume after a
 // rebuild between steps.
+template <bool TM>
 __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int bonded_wg0) {
     __shared__ float red[6][SORT_WAVES];
     __shared__ unsigned int hist[256];
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int bonded_wg0
     __shared__ unsigned int wave_cnt[SORT_WAVES][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if ((int)blockIdx.x >= bonded_wg0) {
-        bonded_term<false>(P, PRISM_MD_TERM_ALL, ((int)blockIdx.x - bonded_wg0) * SORT_THREADS + tid);
+        bonded_term<false, TM>(P, PRISM_MD_TERM_ALL, ((int)blockIdx.x - bonded_wg0) * SORT_THREADS + tid);
         return;
     }
     if (P.counters->any_rebuild == 0) return;  // nothing to rebuild this step: one load
@@ -432,7 +433,8 @@ void launch_sort(const DevPtrs &P, bool with_bonded, hipStream_t st) {
     const int wg0 = P.n_sys + refresh;
     const int terms = with_bonded ? P.n_bonds + P.n_angles + P.n_dih + P.n_exc : 0;
     const int bonded = (terms + SORT_THREADS - 1) / SORT_THREADS;
-    hipLaunchKernelGGL(k_sort, dim3(wg0 + bonded), dim3(SORT_THREADS), 0, st, P, wg0);
+    if (P.term_major) hipLaunchKernelGGL(k_sort<true>, dim3(wg0 + bonded), dim3(SORT_THREADS), 0, st, P, wg0);
+    else hipLaunchKernelGGL(k_sort<false>, dim3(wg0 + bonded), dim3(SORT_THREADS), 0, st, P, wg0);
 }
 
 void launch_rows(const DevPtrs &P, hipStream_t st) {

@@ -91,7 +91,7 @@ __device__ inline long long wave_xor_ll(long long v, int m) {
 // (the same fixed-point terms, summed in another order), and the step loses
 // a kernel and the partials' round trip through HBM.  The accumulators are
 // zeroed by the integrator's first half when it consumes the carried force.
-template <bool ENERGY, bool DIRECT>
+template <bool ENERGY, bool DIRECT, bool TM>
 __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int il = lane & (SUB - 1), js = lane >> 4;
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 long long bs[3] = {0, 0, 0};
                 if (ai >= 0) {
                     for (int j = P.inc_off[ai] + js; j < P.inc_off[ai + 1]; j += 4) {
-                        const float *cb = P.contrib + (size_t)j * 3;
+                        const float *cb = P.contrib + (size_t)(TM ? P.inc_idx[j] : j) * 3;
                         bs[0] += to_fixed(cb[0], FORCE_FIXED_BITS);
                         bs[1] += to_fixed(cb[1], FORCE_FIXED_BITS);
                         bs[2] += to_fixed(cb[2], FORCE_FIXED_BITS);
@@ -239,9 +239,9 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     }
 }
 
-template <bool ENERGY>
+template <bool ENERGY, bool TM>
 __global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
-    bonded_term<ENERGY>(P, mask, blockIdx.x * blockDim.x + threadIdx.x);
+    bonded_term<ENERGY, TM>(P, mask, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // One wave per block quarter (16 sorted slots), four lanes per slot: lane
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_bonded(DevPtrs P, unsigned mask) {
 // lanes' sums are then combined.  Integer addition is associative, so the
 // force is bitwise independent of this split.  For a given k the 16 lanes of a
 // quarter read 256 contiguous bytes of each unit partial.
-template <bool ENERGY>
+template <bool ENERGY, bool TM>
 __global__ __launch_bounds__(256) void k_gather(DevPtrs P, unsigned mask) {
     const int lane = threadIdx.x & 63;
     const int il = lane & (SUB - 1), k = lane >> 4;
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs P, unsigned mask) {
             }
         }
         for (int j = P.inc_off[a] + k; j < P.inc_off[a + 1]; j += 4) {
-            const float *c = P.contrib + (size_t)j * 3;
+            const float *c = P.contrib + (size_t)(TM ? P.inc_idx[j] : j) * 3;
             acc[0] += to_fixed(c[0], FORCE_FIXED_BITS);
             acc[1] += to_fixed(c[1], FORCE_FIXED_BITS);
             acc[2] += to_fixed(c[2], FORCE_FIXED_BITS);
@@ -406,49 +406,62 @@ __global__ __launch_bounds__(256) void k_pairs(DevPtrs P, int mode, RefGrid g, i
 
 // Workgroups of a nonbonded kernel variant that the device holds at once
 // (occupancy x compute units), queried once per device.
-template <bool ENERGY, bool DIRECT> static int resident_blocks() {
+template <bool ENERGY, bool DIRECT, bool TM> static int resident_blocks() {
     static int cache[64];
     int dev = 0;
     (void)hipGetDevice(&dev);
     int &c = cache[dev & 63];
     if (c == 0) {
         int per_cu = 0, cus = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbonded<ENERGY, DIRECT>, 64 * NB_WAVES, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbonded<ENERGY, DIRECT, TM>, 64 * NB_WAVES, 0);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         c = max(1, per_cu) * max(1, cus);
     }
     return c;
 }
 
-template <bool ENERGY, bool DIRECT> static void launch_nb(const DevPtrs &P, hipStream_t st) {
+template <bool ENERGY, bool DIRECT, bool TM> static void launch_nb(const DevPtrs &P, hipStream_t st) {
     // persistent grid: every wave resident at once (no second round of
     // workgroups), but no more waves than a typical unit list needs
     const int want = (P.n_blocks_arena * QUARTERS * 12 + NB_WAVES - 1) / NB_WAVES;
-    const int res = resident_blocks<ENERGY, DIRECT>();
+    const int res = resident_blocks<ENERGY, DIRECT, TM>();
     int grid = max(1, min(want, res));
     if (res % N_XCD == 0) grid = (grid + N_XCD - 1) / N_XCD * N_XCD;  // whole XCD rounds
-    hipLaunchKernelGGL((k_nonbonded<ENERGY, DIRECT>), dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
+    hipLaunchKernelGGL((k_nonbonded<ENERGY, DIRECT, TM>), dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
 }
 
 void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st) {
-    if (energy) launch_nb<true, false>(P, st);
-    else launch_nb<false, false>(P, st);
+    if (energy) launch_nb<true, false, false>(P, st);  // stores unit partials: no bonded slots read
+    else launch_nb<false, false, false>(P, st);
 }
 
-void launch_nonbonded_direct(const DevPtrs &P, hipStream_t st) { launch_nb<false, true>(P, st); }
+void launch_nonbonded_direct(const DevPtrs &P, hipStream_t st) {
+    if (P.term_major) launch_nb<false, true, true>(P, st);
+    else launch_nb<false, true, false>(P, st);
+}
 
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t st) {
     const int n = P.n_bonds + P.n_angles + P.n_dih + P.n_exc;
     if (n == 0) return;
     const int grid = (n + 255) / 256;
-    if (energy) hipLaunchKernelGGL(k_bonded<true>, dim3(grid), dim3(256), 0, st, P, term_mask);
-    else hipLaunchKernelGGL(k_bonded<false>, dim3(grid), dim3(256), 0, st, P, term_mask);
+    if (P.term_major) {
+        if (energy) hipLaunchKernelGGL((k_bonded<true, true>), dim3(grid), dim3(256), 0, st, P, term_mask);
+        else hipLaunchKernelGGL((k_bonded<false, true>), dim3(grid), dim3(256), 0, st, P, term_mask);
+    } else {
+        if (energy) hipLaunchKernelGGL((k_bonded<true, false>), dim3(grid), dim3(256), 0, st, P, term_mask);
+        else hipLaunchKernelGGL((k_bonded<false, false>), dim3(grid), dim3(256), 0, st, P, term_mask);
+    }
 }
 
 void launch_gather(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t st) {
     const int grid = (P.n_atoms_arena / SUB * 64 + 255) / 256;  // one wave per block quarter
-    if (energy) hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(256), 0, st, P, term_mask);
-    else hipLaunchKernelGGL(k_gather<false>, dim3(grid), dim3(256), 0, st, P, term_mask);
+    if (P.term_major) {
+        if (energy) hipLaunchKernelGGL((k_gather<true, true>), dim3(grid), dim3(256), 0, st, P, term_mask);
+        else hipLaunchKernelGGL((k_gather<false, true>), dim3(grid), dim3(256), 0, st, P, term_mask);
+    } else {
+        if (energy) hipLaunchKernelGGL((k_gather<true, false>), dim3(grid), dim3(256), 0, st, P, term_mask);
+        else hipLaunchKernelGGL((k_gather<false, false>), dim3(grid), dim3(256), 0, st, P, term_mask);
+    }
 }
 
 void launch_energy_reduce(const DevPtrs &P, hipStream_t st) {

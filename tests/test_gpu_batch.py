@@ -80,3 +80,27 @@ def test_batch_forces_match_oracle(pm, oracle, systems):
             Fo, Eo = oracle.forces(top.with_positions(starts[s]), starts[s].astype(np.float64), 31, True)
             r = np.sqrt(((F - Fo) ** 2).sum() / (Fo ** 2).sum())
             assert r <= 1e-5, (s, r)
+
+
+def test_large_arena_contribution_layout_matches_alone(pm, systems):
+    """A batch of 21 x 5,200 atoms (an arena above 100,000 atoms) stores the
+    bonded contributions term-major, a system alone atom-major; the forces
+    are the same integers and the trajectories stay bitwise equal."""
+    tops, starts, _ = systems
+    top, x0 = tops[3], starts[3]
+    n = 21
+    params = [pm.default_params(seed=300 + k, stream_id=k) for k in range(n)]
+    with pm.Ensemble([top] * n, params) as ens:
+        for s in range(n):
+            ens.set_state(s, x=x0)
+        fb = ens.compute_forces(5)[0]
+        ens.step(60, graph_steps=10)
+        for s in (0, 5, 20):
+            with pm.FusedEngine(top, params[s]) as eng:
+                eng.set_state(x=x0)
+                if s == 5:
+                    fa, _ = eng.compute_forces()
+                    assert np.array_equal(fa, fb)
+                eng.step(60)
+                assert np.array_equal(eng.get_positions(), ens.get_positions(s)), s
+                assert np.array_equal(eng.get_velocities(), ens.get_velocities(s)), s
