@@ -47,6 +47,7 @@ __device__ inline void put(const DevPtrs &P, int slot, double x, double y, doubl
 // FP32 evaluation to ~1e-5 relative.
 template <bool ENERGY, bool TM>
 __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
+    if (tid >= P.n_bonds + P.n_angles + P.n_dih + P.n_exc || !P.term_live[tid / TERM_CHUNK]) return;
     int idx = tid;
     double energy = 0.0;
     if (idx < P.n_bonds) {

@@ -295,6 +295,8 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     d_dst_b_ = dalloc<int>(2 * NB); d_dst_a_ = dalloc<int>(3 * NA);
     d_dst_d_ = dalloc<int>(4 * ND); d_dst_e_ = dalloc<int>(2 * NE);
     d_inc_idx_ = dalloc<int>(std::max<size_t>(CS, 1));
+    term_live_.assign((NB + NA + ND + NE) / TERM_CHUNK + 1, 0);
+    d_term_live_ = dalloc<unsigned char>(term_live_.size());
     P_.term_e = dalloc<double>(n_terms);
     P_.atom_e = dalloc<double>(A);
     d_skip_off_ = dalloc<int>(A + 1); d_skip_idx_ = dalloc<int>(h.skip_idx.size());
@@ -329,6 +331,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     P_.inc_off = d_inc_off_;
     P_.dst_bond = d_dst_b_; P_.dst_angle = d_dst_a_; P_.dst_dih = d_dst_d_; P_.dst_exc = d_dst_e_;
     P_.inc_idx = d_inc_idx_;
+    P_.term_live = d_term_live_;
     // the contribution layout: term-major from 100,000 arena atoms (measured:
     // +3 % at 250k atoms, +7 % for the 1.33 M-atom ensemble, -1 % at 25k)
     P_.term_major = A >= 100000 ? 1 : 0;
@@ -694,6 +697,7 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
         if (int rc = lower(s, systems[s].topo, systems[s].params)) return rc;
         if (int rc = upload_slot(s)) return rc;
     }
+    if (int rc = upload_term_live()) return rc;
     launch_kinetic(P_, d_red_, -2, stream_);  // initial COM velocity (zero by construction)
     if (int rc = refresh_all()) return rc;
     return check_status();
@@ -702,6 +706,7 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
 int Engine::init_pool(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     if (caps.empty()) return fail(PRISM_MD_E_INVALID, "no slots");
     if (int rc = layout(caps, stream)) return rc;
+    if (int rc = upload_term_live()) return rc;  // no system yet: every chunk off
     return check_status();
 }
 
@@ -716,6 +721,7 @@ int Engine::admit(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
         return rc;
     }
     if (int rc = upload_slot(s)) return rc;
+    if (int rc = upload_term_live()) return rc;
     launch_kinetic(P_, d_red_, s, stream_);
     if (int rc = refresh(s)) return rc;
     return check_status();
@@ -1182,6 +1188,25 @@ int Engine::stats(int s, PrismMdStatsV1 *out) {
     return PRISM_MD_OK;
 }
 
+// Live flags of the bonded term chunks: a chunk is computed when one of its
+// terms belongs to an active system (the term ranges in each descriptor).
+int Engine::upload_term_live() {
+    std::fill(term_live_.begin(), term_live_.end(), 0);
+    const long long sec[4] = {0, P_.n_bonds, (long long)P_.n_bonds + P_.n_angles,
+                              (long long)P_.n_bonds + P_.n_angles + P_.n_dih};
+    for (const SysDesc &D : desc_) {
+        if (!D.active) continue;
+        const int begin[4] = {D.bond_begin, D.angle_begin, D.dih_begin, D.exc_begin};
+        const int count[4] = {D.n_bonds, D.n_angles, D.n_dih, D.n_exc};
+        for (int k = 0; k < 4; ++k)
+            if (count[k] > 0)
+                for (long long c = (sec[k] + begin[k]) / TERM_CHUNK; c <= (sec[k] + begin[k] + count[k] - 1) / TERM_CHUNK; ++c)
+                    term_live_[(size_t)c] = 1;
+    }
+    return hipcheck(hipMemcpyAsync(d_term_live_, term_live_.data(), term_live_.size(), hipMemcpyHostToDevice, stream_),
+                    "term flags");
+}
+
 int Engine::set_active(int s, bool active) {
     if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
     if (sys_[s].n_atoms == 0) return fail(PRISM_MD_E_INVALID, "slot holds no system");
@@ -1190,6 +1215,7 @@ int Engine::set_active(int s, bool active) {
     if (int rc = hipcheck(hipMemcpyAsync(d_desc_ + s, &desc_[s], sizeof(SysDesc), hipMemcpyHostToDevice, stream_),
                           "activate"))
         return rc;
+    if (int rc = upload_term_live()) return rc;
     if (active) {
         // re-sort and rebuild this system, re-claim the unit list, and give it
         // its carried force before its first step

@@ -126,6 +126,9 @@ class Engine {
     int *d_atom_sys_ = nullptr, *d_block_sys_ = nullptr, *d_wave_sys_ = nullptr;
     int *d_skip_off_ = nullptr, *d_skip_idx_ = nullptr, *d_ex_off_ = nullptr, *d_ex_idx_ = nullptr, *d_inc_off_ = nullptr;
     int *d_dst_b_ = nullptr, *d_dst_a_ = nullptr, *d_dst_d_ = nullptr, *d_dst_e_ = nullptr, *d_inc_idx_ = nullptr;
+    unsigned char *d_term_live_ = nullptr;
+    std::vector<unsigned char> term_live_;  // host copy of P_.term_live
+    int upload_term_live();
     Group *d_groups_ = nullptr;
     GroupParam *d_gp_ = nullptr;
     Bond *d_bonds_ = nullptr;

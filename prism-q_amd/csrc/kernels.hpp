@@ -42,6 +42,11 @@ struct DevPtrs {
     const int *dst_bond, *dst_angle, *dst_dih, *dst_exc;  // atom-major slot per term position: [n][arity]
     const int *inc_idx;             // term-major: [n_slots] slots of each atom
     int term_major;
+    // per TERM_CHUNK consecutive terms (bonds, angles, dihedrals, exceptions
+    // concatenated): 1 when one of them belongs to an active system.  The
+    // bonded threads of other chunks return at once -- a serving pool's
+    // arena is mostly free slots and padding.
+    const unsigned char *term_live;
     double *term_e;                 // per-term energy (bonds, angles, dihedrals, exceptions)
     double *atom_e;                 // per-atom nonbonded energy of the last gather
     unsigned long long *masks;      // [B][MAX_MASKS][64] per partner block: skip bits of each i-lane

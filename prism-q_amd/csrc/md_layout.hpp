@@ -28,6 +28,7 @@ constexpr int CHUNKS_PER_ROW = MAX_ROW / CHUNK_MIN;
 constexpr int UNITS_PER_BLOCK = QUARTERS * CHUNKS_PER_ROW;
 constexpr int FORCE_FIXED_BITS = 24; // int64 fixed point: 2^24 per kcal/mol/A
 constexpr int ENERGY_FIXED_BITS = 24;
+constexpr int TERM_CHUNK = 256;     // terms per live flag (term_live)
 constexpr float DUMMY_FAR = 1.0e7f;  // padding slots sit far from every atom
 
 constexpr double COULOMB = 332.0636;         // nhs_amber_fused.cu:66
