@@ -317,6 +317,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     P_.com_partial = dalloc<double>((size_t)W * 3);
     P_.sweeps = dalloc<int>((size_t)W * 2);
     P_.counters = dalloc<EngineCounters>(1);
+    P_.hs_group = dalloc<unsigned>((size_t)W / 64 + 2);  // >= second-half workgroups / 128, zeroed
     d_count_ = dalloc<unsigned long long>(1);
     d_red_ = dalloc<double>((size_t)n_sys * 4);
     d_noise_ = dalloc<float>((size_t)2 * A * 3);

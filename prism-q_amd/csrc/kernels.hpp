@@ -47,6 +47,7 @@ struct DevPtrs {
     // bonded threads of other chunks return at once -- a serving pool's
     // arena is mostly free slots and padding.
     const unsigned char *term_live;
+    unsigned *hs_group;             // [second-half workgroups / HS_GROUP] step-end handshake counters
     double *term_e;                 // per-term energy (bonds, angles, dihedrals, exceptions)
     double *atom_e;                 // per-atom nonbonded energy of the last gather
     unsigned long long *masks;      // [B][MAX_MASKS][64] per partner block: skip bits of each i-lane

@@ -19,6 +19,8 @@ namespace prism {
 namespace {
 
 constexpr int IW = 4;  // waves per workgroup
+constexpr int HS_GROUP = 128;  // workgroups per first-level counter of the step-end handshake
+constexpr unsigned HS_PUBLISHED = 1u << 16;  // count in the low 16 bits, publishers above
 
 // Register-resident group state.  Every loop over the group's atoms or bonds
 // below is fully unrolled with a runtime guard, so all indices are
@@ -421,10 +423,34 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
     // A workgroup releases its writes (agent-scope fence) only when the last
     // one reads any of them -- COM partials, a latched fault: the fence writes
     // back this XCD's L2 and costs microseconds per workgroup, so on the other
-    // steps the counter increment goes out unfenced.
-    if (__syncthreads_or(publish)) __threadfence();
+    // steps the counter increment goes out unfenced.  The count has two
+    // levels, HS_GROUP workgroups per group counter and one increment per
+    // group on the global one: atomics on one address serialise (~12 ns
+    // each), which a flat count over the thousands of workgroups of a large
+    // arena would pay on every step.  A workgroup that published adds
+    // HS_PUBLISHED, so the last of its group knows to pass the writes on.
+    const int pub = __syncthreads_or(publish);
+    if (pub) __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&P.counters->done_blocks, 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) {
+        const unsigned g = blockIdx.x / HS_GROUP;
+        const unsigned gsize = min((unsigned)HS_GROUP, gridDim.x - g * HS_GROUP);
+        const unsigned n_groups = (gridDim.x + HS_GROUP - 1) / HS_GROUP;
+        const unsigned old = atomicAdd(&P.hs_group[g], pub ? HS_PUBLISHED + 1u : 1u);
+        int is_last = 0;
+        if ((old & (HS_PUBLISHED - 1u)) == gsize - 1) {  // the group's last workgroup
+            P.hs_group[g] = 0u;  // for the next launch; no other workgroup touches it now
+            if (n_groups == 1) {  // a small arena: one level
+                is_last = 1;
+            } else {
+                const bool gpub = pub || old >= HS_PUBLISHED;
+                if (gpub) __threadfence();
+                const unsigned o2 = atomicAdd(&P.counters->done_groups, gpub ? HS_PUBLISHED + 1u : 1u);
+                is_last = (o2 & (HS_PUBLISHED - 1u)) == n_groups - 1;
+            }
+        }
+        last = is_last;
+    }
     __syncthreads();
     if (!last) return;
     __threadfence();
@@ -451,7 +477,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         if (S.status & ST_HALT) S.halted = 1;
     }
     if (threadIdx.x == 0) {
-        P.counters->done_blocks = 0;
+        P.counters->done_groups = 0u;
         P.counters->any_dirty = 0;
         P.counters->any_rebuild = 0;
     }

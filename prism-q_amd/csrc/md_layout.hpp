@@ -116,7 +116,8 @@ __host__ __device__ inline bool resort_due(const SysDesc &D, const SysState &S) 
 
 // Global engine counters (device-written).
 struct EngineCounters {
-    unsigned int done_blocks;       // last-block handshake of the integrator
+    unsigned int done_groups;       // step-end handshake of the integrator: groups of workgroups
+                                    // done (low 16 bits) + 1 << 16 per group that published
     int any_dirty;                  // some system rebuilds its rows this step
     int n_units;                    // nonbonded work units claimed by the row builder
     int any_rebuild;                // some system raised its rebuild flag (cleared with any_dirty);
