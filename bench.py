@@ -200,7 +200,8 @@ def copy_calibration(mib: int, reps: int = 5):
     log(f"copy calibration: {reps} device-to-device copies of {mib} MiB")
 
 
-def ensemble_leg(pm, n_systems, steps, graph_steps, tune_buckets, chunk_tiles=0, seed=0x50524953 + 3):
+def ensemble_leg(pm, n_systems, steps, graph_steps, tune_buckets, chunk_tiles=0, plain_runs=True,
+                 seed=0x50524953 + 3):
     """Batched ensemble (one launch per phase over all systems, captured
     hipGraph) against the same systems run one at a time on the plain path.
     Sizes log-uniform in [500, 20000] atoms, temperatures uniform in
@@ -267,7 +268,7 @@ def ensemble_leg(pm, n_systems, steps, graph_steps, tune_buckets, chunk_tiles=0,
     wall_p = 0.0
     plain_lat = []
     plain_atoms = 0
-    for k in range(n_systems):
+    for k in range(n_systems if plain_runs else 0):
         if k in bad:
             continue
         plain_atoms += int(sizes[k])
@@ -279,14 +280,14 @@ def ensemble_leg(pm, n_systems, steps, graph_steps, tune_buckets, chunk_tiles=0,
             dt = time.perf_counter() - t1
             wall_p += dt
             plain_lat.append(dt / done_b)
-    plain = plain_atoms * done_b / wall_p
+    plain = plain_atoms * done_b / wall_p if wall_p > 0 else None
     lat = np.array(lat) * 1e3
     return {"n_systems": n_systems, "atoms": total, "steps": done_b, "graph_steps": graph_steps,
-            "atom_steps_per_s": round(batched, 1), "plain_atom_steps_per_s": round(plain, 1),
-            "speedup_vs_plain": round(batched / plain, 3), "chunk_tiles": chunk_tiles,
+            "atom_steps_per_s": round(batched, 1), "plain_atom_steps_per_s": round(plain, 1) if plain else None,
+            "speedup_vs_plain": round(batched / plain, 3) if plain else None, "chunk_tiles": chunk_tiles,
             "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                                 "p99": round(float(np.percentile(lat, 99)), 4)},
-            "plain_step_latency_ms_all_systems_sequential": round(float(sum(plain_lat)) * 1e3, 4),
+            "plain_step_latency_ms_all_systems_sequential": round(float(sum(plain_lat)) * 1e3, 4) if plain_lat else None,
             "systems_with_faults": bad,
             "autotune": [{k: e[k] for k in ("lo", "hi", "sort_cell_A", "resort_every", "us_per_step")} for e in table],
             "sizes": "log-uniform 500-20000 atoms", "temperatures_K": "uniform 280-340"}
@@ -400,6 +401,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=400)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--tune-buckets", type=int, default=4, help="autotuner size buckets for the ensemble (0 = defaults)")
+    ap.add_argument("--ensemble-no-plain", action="store_true",
+                    help="skip the one-at-a-time baseline of the ensemble (for profiling the batched steps)")
     ap.add_argument("--ensemble-chunk-tiles", type=int, default=16,
                     help="row tiles per nonbonded work unit in the batched ensemble (0 = the engine's choice per "
                          "system; 16 measured 3 %% faster than that for the 256-system batch)")
@@ -471,7 +474,7 @@ def main():
     out["ensemble"] = None
     if args.ensemble_systems > 0:
         out["ensemble"] = ensemble_leg(pm, args.ensemble_systems, args.ensemble_steps, args.graph_steps,
-                                       args.tune_buckets, args.ensemble_chunk_tiles)
+                                       args.tune_buckets, args.ensemble_chunk_tiles, not args.ensemble_no_plain)
         log(f"ensemble: {out['ensemble']}")
     out["large_system"] = None
     if args.large_atoms > 0:
