@@ -6,6 +6,13 @@
 
 namespace prism {
 
+// The wave's index in its workgroup, as a value the compiler knows to be the
+// same on every lane: from threadIdx.x >> 6 it cannot tell, so whatever is
+// derived from it (a system index, its descriptor and state, a unit code)
+// would be read with vector loads and branched on under the exec mask rather
+// than with scalar loads and scalar branches.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 __device__ inline int wave_min_int(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
     return v;
@@ -83,16 +90,22 @@ __device__ inline void langevin_normals(uint32_t seed_lo, uint32_t stream, uint3
 // Director schedule (protocol_director.cu:58-123): temperature table and the
 // equilibration friction ramp at schedule step `step`.
 __device__ inline void schedule(const SysDesc &D, unsigned long long step, float &T, float &gamma) {
+    // every field is read before the first comparison, so the loads issue
+    // together instead of one per branch taken
     const long long s = (long long)step;
-    if (s < D.cold_hold_end) T = D.t_start;
-    else if (s < D.ramp_end) T = D.t_start + (float)(s - D.cold_hold_end) / (float)(D.ramp_end - D.cold_hold_end) * (D.t_end - D.t_start);
-    else if (s < D.warm_hold_end) T = D.t_end;
-    else if (s < D.ramp_down_end) T = D.t_end - (float)(s - D.warm_hold_end) / (float)(D.ramp_down_end - D.warm_hold_end) * (D.t_end - D.t_start);
-    else T = D.warm_tail ? D.t_end : D.t_start;
-    gamma = D.gamma_base;
-    if (s < D.equilibration_steps) {
-        const float decay = expf(-3.0f * (float)s / (float)D.equilibration_steps);
-        gamma = D.gamma_eq * decay + D.gamma_base * (1.0f - decay);
+    const long long e_cold = D.cold_hold_end, e_ramp = D.ramp_end, e_warm = D.warm_hold_end, e_down = D.ramp_down_end;
+    const long long e_eq = D.equilibration_steps;
+    const float t0 = D.t_start, t1 = D.t_end, g_base = D.gamma_base, g_eq = D.gamma_eq;
+    const bool warm_tail = D.warm_tail;
+    if (s < e_cold) T = t0;
+    else if (s < e_ramp) T = t0 + (float)(s - e_cold) / (float)(e_ramp - e_cold) * (t1 - t0);
+    else if (s < e_warm) T = t1;
+    else if (s < e_down) T = t1 - (float)(s - e_warm) / (float)(e_down - e_warm) * (t1 - t0);
+    else T = warm_tail ? t1 : t0;
+    gamma = g_base;
+    if (s < e_eq) {
+        const float decay = expf(-3.0f * (float)s / (float)e_eq);
+        gamma = g_eq * decay + g_base * (1.0f - decay);
     }
 }
 

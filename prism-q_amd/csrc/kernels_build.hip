@@ -114,7 +114,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int bonded_wg0
     __shared__ unsigned int hist[256];
     __shared__ unsigned int running[256];
     __shared__ unsigned int wave_cnt[SORT_WAVES][256];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     if ((int)blockIdx.x >= bonded_wg0) {
         bonded_term<false, TM>(P, PRISM_MD_TERM_ALL, ((int)blockIdx.x - bonded_wg0) * SORT_THREADS + tid);
         return;
@@ -384,7 +384,7 @@ constexpr int ROW_WAVES = 2;
 // order varies, but no result depends on it: the gather reads each block's
 // units through unit_off in chunk order.
 __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = wave_id(), lane = threadIdx.x & 63;
     const int b = blockIdx.x * ROW_WAVES + wave;
     __shared__ int row_lds[ROW_WAVES][MAX_ROW];
     __shared__ int flag_lds[ROW_WAVES][MAX_ROW];

@@ -93,7 +93,7 @@ __device__ inline long long wave_xor_ll(long long v, int m) {
 // zeroed by the integrator's first half when it consumes the carried force.
 template <bool ENERGY, bool DIRECT, bool TM>
 __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = wave_id(), lane = threadIdx.x & 63;
     const int il = lane & (SUB - 1), js = lane >> 4;
     __shared__ float4 jpos_s[NB_WAVES][64];
     __shared__ float4 jpar_s[NB_WAVES][64];
@@ -115,125 +115,135 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
         u_end = min(n_units, (xcd + 1) * span_u);
         stride = (gridDim.x / N_XCD) * NB_WAVES;
     }
-    // the next unit's code is loaded one unit ahead
-    int code_next = u0 < u_end ? P.units[u0] : 0;
-    for (int u = u0; u < u_end; u += stride) {
-        const int code = code_next;
-        if (u + stride < u_end) code_next = P.units[u + stride];
-        const int b = code >> 11, q = (code >> 9) & 3, t0 = code & (MAX_ROW - 1);
-        // tile indices of up to CHUNK_MAX tiles, loaded before the row length
-        // and the descriptor arrive (entries past the row are not used)
-        const int ti = min(t0 + (lane & (CHUNK_MAX - 1)), MAX_ROW - 1);
-        const int my_jb = P.row_j[(size_t)b * MAX_ROW + ti];
-        const int my_ms = P.row_m[(size_t)b * MAX_ROW + ti];
-        const int s = P.block_sys[b];
-        const SysDesc &D = P.sys[s];
-        const int nrow = P.row_n[b];
-        const int t1 = min(nrow, t0 + D.chunk);
-        const int c = t0 == 0 ? 0 : 1;  // first unit of the quarter or not
-        const float reach = D.cutoff + 0.001f;
-        const float reach2 = reach * reach;
-        const unsigned lo_bits = __float_as_uint(0.01f);
-        const unsigned span = __float_as_uint(D.cutoff2) - lo_bits;
-        const float inv_rc2 = 1.0f / D.cutoff2, inv_rc = rsqrtf(D.cutoff2);
+    // The codes of the wave's next 64 units arrive in one load, one per lane
+    // (a wave seldom runs more than a few units), and each unit reads its code
+    // from its lane.  The inner loop issues no load that a later unit waits on,
+    // so a unit never waits for the previous unit's force atomics to retire.
+    for (int ub = u0; ub < u_end; ub += 64 * stride) {
+        const int uu = ub + lane * stride;
+        const int codes = uu < u_end ? P.units[uu] : 0;
+        const int n_here = min(64, (u_end - ub + stride - 1) / stride);
+        // read before the loop, so the wait for the codes happens here and
+        // not at the loop head, where it would also wait for the atomics
+        const int code0 = __builtin_amdgcn_readlane(codes, 0);
+        for (int it = 0; it < n_here; ++it) {
+            const int u = ub + it * stride;
+            const int code = it == 0 ? code0 : __builtin_amdgcn_readlane(codes, it);
+            const int b = code >> 11, q = (code >> 9) & 3, t0 = code & (MAX_ROW - 1);
+            // tile indices of up to CHUNK_MAX tiles, loaded before the row length
+            // and the descriptor arrive (entries past the row are not used)
+            const int ti = min(t0 + (lane & (CHUNK_MAX - 1)), MAX_ROW - 1);
+            const int my_jb = P.row_j[(size_t)b * MAX_ROW + ti];
+            const int my_ms = P.row_m[(size_t)b * MAX_ROW + ti];
+            const int s = P.block_sys[b];
+            const SysDesc &D = P.sys[s];
+            const int nrow = P.row_n[b];
+            const int t1 = min(nrow, t0 + D.chunk);
+            const int c = t0 == 0 ? 0 : 1;  // first unit of the quarter or not
+            const float reach = D.cutoff + 0.001f;
+            const float reach2 = reach * reach;
+            const unsigned lo_bits = __float_as_uint(0.01f);
+            const unsigned span = __float_as_uint(D.cutoff2) - lo_bits;
+            const float inv_rc2 = 1.0f / D.cutoff2, inv_rc = rsqrtf(D.cutoff2);
 
-        const int slot_i = b * BLOCK + q * SUB + il;
-        const float4 xi = P.xq[slot_i];
-        const float2 li = P.lj[slot_i];
-        const int ai = P.perm[slot_i];
-        float lo[3], hi[3];
-        if (ai >= 0) {
-            lo[0] = hi[0] = xi.x; lo[1] = hi[1] = xi.y; lo[2] = hi[2] = xi.z;
-        } else {
-            lo[0] = lo[1] = lo[2] = 3.0e38f;
-            hi[0] = hi[1] = hi[2] = -3.0e38f;
-        }
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = wave_min(lo[k]);
-            hi[k] = wave_max(hi[k]);
-        }
-        const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
-        const float hx = 0.5f * (hi[0] - lo[0]), hy = 0.5f * (hi[1] - lo[1]), hz = 0.5f * (hi[2] - lo[2]);
+            const int slot_i = b * BLOCK + q * SUB + il;
+            const float4 xi = P.xq[slot_i];
+            const float2 li = P.lj[slot_i];
+            const int ai = P.perm[slot_i];
+            float lo[3], hi[3];
+            if (ai >= 0) {
+                lo[0] = hi[0] = xi.x; lo[1] = hi[1] = xi.y; lo[2] = hi[2] = xi.z;
+            } else {
+                lo[0] = lo[1] = lo[2] = 3.0e38f;
+                hi[0] = hi[1] = hi[2] = -3.0e38f;
+            }
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = wave_min(lo[k]);
+                hi[k] = wave_max(hi[k]);
+            }
+            const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
+            const float hx = 0.5f * (hi[0] - lo[0]), hy = 0.5f * (hi[1] - lo[1]), hz = 0.5f * (hi[2] - lo[2]);
 
-        // the unit's tile indices and mask slots, one per lane, then a software
-        // pipeline: the next tile's j-atoms load while this tile computes
-        const int n_t = t1 - t0;
-        int jb = __shfl(my_jb, 0);
-        float4 xj = P.xq[jb * BLOCK + lane];
-        float2 lj = P.lj[jb * BLOCK + lane];
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
-        for (int k = 0; k < n_t; ++k) {
-            const int ms = __shfl(my_ms, k);
-            float4 xn = xj;
-            float2 ln = lj;
-            if (k + 1 < n_t) {
-                const int jn = __shfl(my_jb, k + 1);
-                xn = P.xq[jn * BLOCK + lane];
-                ln = P.lj[jn * BLOCK + lane];
+            // the unit's tile indices and mask slots, one per lane, then a software
+            // pipeline: the next tile's j-atoms load while this tile computes
+            const int n_t = t1 - t0;
+            int jb = __shfl(my_jb, 0);
+            float4 xj = P.xq[jb * BLOCK + lane];
+            float2 lj = P.lj[jb * BLOCK + lane];
+            float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
+            for (int k = 0; k < n_t; ++k) {
+                const int ms = __shfl(my_ms, k);
+                float4 xn = xj;
+                float2 ln = lj;
+                if (k + 1 < n_t) {
+                    const int jn = __shfl(my_jb, k + 1);
+                    xn = P.xq[jn * BLOCK + lane];
+                    ln = P.lj[jn * BLOCK + lane];
+                }
+                const float ex = fmaxf(fabsf(xj.x - cx) - hx, 0.0f);
+                const float ey = fmaxf(fabsf(xj.y - cy) - hy, 0.0f);
+                const float ez = fmaxf(fabsf(xj.z - cz) - hz, 0.0f);
+                const bool active = ex * ex + ey * ey + ez * ez <= reach2;
+                const unsigned long long m = __ballot(active);
+                const int n_act = __popcll(m);
+                if (n_act > 0) {
+                    if (active) {
+                        const int pos = __popcll(m & lt);
+                        jpos[pos] = xj;
+                        jpar[pos] = make_float4(lj.x, lj.y, __int_as_float(lane), 0.0f);
+                    }
+                    const int n_pad = (n_act + 3) & ~3;
+                    if (lane >= n_act && lane < n_pad) {  // dummies: r2 ~ 1e14, outside every cutoff
+                        jpos[lane] = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f);
+                        jpar[lane] = make_float4(0.0f, 0.0f, __int_as_float(0), 0.0f);
+                    }
+                    wave_lds_fence();
+                    const int n_iter = n_pad >> 2;
+                    if (ms >= 0) {
+                        const unsigned long long excl = P.masks[((size_t)b * MAX_MASKS + ms) * 64 + q * SUB + il];
+                        tile_pairs<ENERGY, true>(xi, li, n_iter, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
+                    } else {
+                        tile_pairs<ENERGY, false>(xi, li, n_iter, js, jpos, jpar, 0ull, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
+                    }
+                    wave_lds_fence();
+                }
+                xj = xn;
+                lj = ln;
             }
-            const float ex = fmaxf(fabsf(xj.x - cx) - hx, 0.0f);
-            const float ey = fmaxf(fabsf(xj.y - cy) - hy, 0.0f);
-            const float ez = fmaxf(fabsf(xj.z - cz) - hz, 0.0f);
-            const bool active = ex * ex + ey * ey + ez * ez <= reach2;
-            const unsigned long long m = __ballot(active);
-            const int n_act = __popcll(m);
-            if (n_act > 0) {
-                if (active) {
-                    const int pos = __popcll(m & lt);
-                    jpos[pos] = xj;
-                    jpar[pos] = make_float4(lj.x, lj.y, __int_as_float(lane), 0.0f);
-                }
-                const int n_pad = (n_act + 3) & ~3;
-                if (lane >= n_act && lane < n_pad) {  // dummies: r2 ~ 1e14, outside every cutoff
-                    jpos[lane] = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f);
-                    jpar[lane] = make_float4(0.0f, 0.0f, __int_as_float(0), 0.0f);
-                }
-                wave_lds_fence();
-                const int n_iter = n_pad >> 2;
-                if (ms >= 0) {
-                    const unsigned long long excl = P.masks[((size_t)b * MAX_MASKS + ms) * 64 + q * SUB + il];
-                    tile_pairs<ENERGY, true>(xi, li, n_iter, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
-                } else {
-                    tile_pairs<ENERGY, false>(xi, li, n_iter, js, jpos, jpar, 0ull, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
-                }
-                wave_lds_fence();
+            // sum the four j-slot partials of each i-atom (lanes il, il+16, il+32,
+            // il+48) in a fixed order and store the unit's partial; the gather
+            // adds the partials of an atom's units in unit order
+            fx += __shfl_xor(fx, 16); fy += __shfl_xor(fy, 16); fz += __shfl_xor(fz, 16);
+            fx += __shfl_xor(fx, 32); fy += __shfl_xor(fy, 32); fz += __shfl_xor(fz, 32);
+            if (ENERGY) {
+                e += __shfl_xor(e, 16);
+                e += __shfl_xor(e, 32);
             }
-            xj = xn;
-            lj = ln;
-        }
-        // sum the four j-slot partials of each i-atom (lanes il, il+16, il+32,
-        // il+48) in a fixed order and store the unit's partial; the gather
-        // adds the partials of an atom's units in unit order
-        fx += __shfl_xor(fx, 16); fy += __shfl_xor(fy, 16); fz += __shfl_xor(fz, 16);
-        fx += __shfl_xor(fx, 32); fy += __shfl_xor(fy, 32); fz += __shfl_xor(fz, 32);
-        if (ENERGY) {
-            e += __shfl_xor(e, 16);
-            e += __shfl_xor(e, 32);
-        }
-        if (!DIRECT) {
-            if (js == 0) P.unit_f[(size_t)u * SUB + il] = make_float4(fx, fy, fz, ENERGY ? e : 0.0f);
-        } else {
-            long long add[3] = {to_fixed(fx, FORCE_FIXED_BITS), to_fixed(fy, FORCE_FIXED_BITS),
-                                to_fixed(fz, FORCE_FIXED_BITS)};
-            if (c == 0) {  // the quarter's bonded slots, lane js taking slots js, js+4, ...
-                long long bs[3] = {0, 0, 0};
-                if (ai >= 0) {
-                    for (int j = P.inc_off[ai] + js; j < P.inc_off[ai + 1]; j += 4) {
-                        const float *cb = P.contrib + (size_t)(TM ? P.inc_idx[j] : j) * 3;
-                        bs[0] += to_fixed(cb[0], FORCE_FIXED_BITS);
-                        bs[1] += to_fixed(cb[1], FORCE_FIXED_BITS);
-                        bs[2] += to_fixed(cb[2], FORCE_FIXED_BITS);
+            if (!DIRECT) {
+                if (js == 0) P.unit_f[(size_t)u * SUB + il] = make_float4(fx, fy, fz, ENERGY ? e : 0.0f);
+            } else {
+                long long add[3] = {to_fixed(fx, FORCE_FIXED_BITS), to_fixed(fy, FORCE_FIXED_BITS),
+                                    to_fixed(fz, FORCE_FIXED_BITS)};
+                if (c == 0) {  // the quarter's bonded slots, lane js taking slots js, js+4, ...
+                    long long bs[3] = {0, 0, 0};
+                    if (ai >= 0) {
+                        for (int j = P.inc_off[ai] + js; j < P.inc_off[ai + 1]; j += 4) {
+                            const float *cb = P.contrib + (size_t)(TM ? P.inc_idx[j] : j) * 3;
+                            bs[0] += to_fixed(cb[0], FORCE_FIXED_BITS);
+                            bs[1] += to_fixed(cb[1], FORCE_FIXED_BITS);
+                            bs[2] += to_fixed(cb[2], FORCE_FIXED_BITS);
+                        }
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        bs[k] += wave_xor_ll(bs[k], 16);
+                        bs[k] += wave_xor_ll(bs[k], 32);
+                        add[k] += bs[k];
                     }
                 }
-                for (int k = 0; k < 3; ++k) {
-                    bs[k] += wave_xor_ll(bs[k], 16);
-                    bs[k] += wave_xor_ll(bs[k], 32);
-                    add[k] += bs[k];
+                if (ai >= 0 && js < 3) {
+                    const long long v = js == 0 ? add[0] : (js == 1 ? add[1] : add[2]);
+                    atomicAdd((unsigned long long *)(P.f + (size_t)ai * 4 + js), (unsigned long long)v);
                 }
-            }
-            if (ai >= 0 && js < 3) {
-                const long long v = js == 0 ? add[0] : (js == 1 ? add[1] : add[2]);
-                atomicAdd((unsigned long long *)(P.f + (size_t)ai * 4 + js), (unsigned long long)v);
             }
         }
     }

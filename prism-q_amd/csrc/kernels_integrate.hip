@@ -30,6 +30,14 @@ struct GroupState {
     int a[4];
     float m[4];
     float x[4][3], v[4][3];
+    long long f[4][3];  // the carried force, raw int64 fixed point
+};
+
+// What the first half reads at its end: each atom's sorted slot and its
+// position at the last list build.
+struct GroupTail {
+    int slot[4];
+    float xb[4][3];
 };
 
 #define FOR_ATOMS(k) _Pragma("unroll") for (int k = 0; k < 4; ++k) if (k < g.n)
@@ -239,11 +247,10 @@ __device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float g
 // evaluation adds into them (first half only).
 __device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, float gamma, bool consume) {
     FOR_ATOMS(k) {
-        long long *fa = P.f + (size_t)g.a[k] * 4;
         float f[3];
-        for (int c = 0; c < 3; ++c) f[c] = from_fixed(fa[c], FORCE_FIXED_BITS);
+        for (int c = 0; c < 3; ++c) f[c] = from_fixed(g.f[k][c], FORCE_FIXED_BITS);
         if (consume)
-            for (int c = 0; c < 3; ++c) fa[c] = 0;
+            for (int c = 0; c < 3; ++c) P.f[(size_t)g.a[k] * 4 + c] = 0;
         if (gamma != 0.0f) {  // |f| > MAX_FORCE tested squared; the rescale is rare
             const float mag2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
             if (mag2 > MAX_FORCE * MAX_FORCE) {
@@ -256,22 +263,38 @@ __device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, floa
     }
 }
 
-__device__ inline bool load_group(const DevPtrs &P, int gidx, GroupState &g, GroupParam &gp) {
+// Load a constraint group and its atoms' state in two rounds of memory
+// traffic: the record and its parameters together (every group index of the
+// arena is in bounds, padding included), then everything each atom needs as
+// soon as its index is known (a group's atom k exists exactly when k <= n_h).
+// The atom loads are unconditional -- an absent atom reads arena atom 0,
+// whose values are never used -- so no branch splits them into one wait per
+// atom.  The integrator kernels run under one wave per SIMD, so each
+// dependent round of loads is exposed latency.  No kernel that calls this
+// writes another lane's atoms, so the early loads see the values the later
+// uses would.
+template <bool TAIL>
+__device__ inline bool load_group(const DevPtrs &P, int gidx, GroupState &g, GroupParam &gp, GroupTail &t) {
     const Group G = P.groups[gidx];
-    if (G.atom[0] < 0) return false;
     gp = P.gparams[gidx];
-    g.n = 1 + gp.n_h;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        g.a[k] = G.atom[k];
-        if (k < g.n) {
-            const int a = G.atom[k];
-            g.m[k] = P.mass[a];
-            const float4 x = P.x[a], v = P.v[a];
-            g.x[k][0] = x.x; g.x[k][1] = x.y; g.x[k][2] = x.z;
-            g.v[k][0] = v.x; g.v[k][1] = v.y; g.v[k][2] = v.z;
+        const int a = G.atom[k];
+        const size_t ac = (size_t)max(a, 0);
+        g.a[k] = a;
+        g.m[k] = P.mass[ac];
+        const float4 x = P.x[ac], v = P.v[ac];
+        g.x[k][0] = x.x; g.x[k][1] = x.y; g.x[k][2] = x.z;
+        g.v[k][0] = v.x; g.v[k][1] = v.y; g.v[k][2] = v.z;
+        for (int c = 0; c < 3; ++c) g.f[k][c] = P.f[ac * 4 + c];
+        if (TAIL) {
+            t.slot[k] = P.slot_of[ac];
+            const float4 b = P.x_build[ac];
+            t.xb[k][0] = b.x; t.xb[k][1] = b.y; t.xb[k][2] = b.z;
         }
     }
+    if (G.atom[0] < 0) return false;
+    g.n = 1 + gp.n_h;
     return true;
 }
 
@@ -299,7 +322,7 @@ __device__ inline void publish_sweeps(int *slot, int prev, int most) {
 
 __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *noise) {
     const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * IW + (threadIdx.x >> 6);
+    const int gw = blockIdx.x * IW + wave_id();
     if (gw >= P.n_waves_arena) return;
     const int s = P.wave_sys[gw];
     if (s < 0) return;
@@ -320,9 +343,10 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
     const int prev_sh = P.sweeps[2 * gw], prev_ra = P.sweeps[2 * gw + 1];
     GroupState g;
     GroupParam gp;
+    GroupTail tail;
     bool moved = false;
     int most_sh = 0, most_ra = 0;
-    if (load_group(P, gw * 64 + lane, g, gp)) {
+    if (load_group<true>(P, gw * 64 + lane, g, gp, tail)) {
         if (com) {
             FOR_ATOMS(k) {
                 for (int c = 0; c < 3; ++c) g.v[k][c] -= (float)S.com_v[c];
@@ -345,10 +369,9 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
             if (!(finite3(g.x[k]) && finite3(g.v[k]))) record_fault(S, FAULT_NONFINITE, a, step);
             P.x[a] = make_float4(g.x[k][0], g.x[k][1], g.x[k][2], 0.0f);
             P.v[a] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
-            float *q = (float *)&P.xq[P.slot_of[a]];
+            float *q = (float *)&P.xq[tail.slot[k]];
             q[0] = g.x[k][0]; q[1] = g.x[k][1]; q[2] = g.x[k][2];
-            const float4 b = P.x_build[a];
-            const float dx = g.x[k][0] - b.x, dy = g.x[k][1] - b.y, dz = g.x[k][2] - b.z;
+            const float dx = g.x[k][0] - tail.xb[k][0], dy = g.x[k][1] - tail.xb[k][1], dz = g.x[k][2] - tail.xb[k][2];
             moved |= dx * dx + dy * dy + dz * dz > D.half_skin2;
         }
     }
@@ -365,7 +388,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
 
 __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float *noise) {
     const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * IW + (threadIdx.x >> 6);
+    const int gw = blockIdx.x * IW + wave_id();
     __shared__ int last;
     bool publish = false;  // this thread wrote something the last workgroup reads
     const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
@@ -379,9 +402,10 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         const int prev_ra = P.sweeps[2 * gw + 1];
         GroupState g;
         GroupParam gp;
+        GroupTail unused;
         double mv[3] = {0.0, 0.0, 0.0};
         int most_ra = 0;
-        if (load_group(P, gw * 64 + lane, g, gp)) {
+        if (load_group<false>(P, gw * 64 + lane, g, gp, unused)) {
             half_kick(g, P, D.dt, gamma, false);
             int sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
             publish |= sw < 0;
