@@ -47,11 +47,17 @@ __device__ inline void put(const DevPtrs &P, int slot, double x, double y, doubl
 // FP32 evaluation to ~1e-5 relative.
 template <bool ENERGY, bool TM>
 __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
-    if (tid >= P.n_bonds + P.n_angles + P.n_dih + P.n_exc || !P.term_live[tid / TERM_CHUNK]) return;
+    if (tid >= P.n_bonds + P.n_angles + P.n_dih + P.n_exc) return;
+    // each branch loads its term record and the chunk's live flag together,
+    // so a live term waits for memory once before its atoms' positions (the
+    // record's first index, never below -1, enters the test so that the
+    // compiler cannot move the record load after it)
+    const unsigned char *live = P.term_live + tid / TERM_CHUNK;
     int idx = tid;
     double energy = 0.0;
     if (idx < P.n_bonds) {
         const Bond B = P.bonds[idx];
+        if ((*live == 0) | (B.i < -1)) return;
         const Slots<TM> dst{P.dst_bond + 2 * idx, 2 * idx};
         double fm = 0.0;
         double3 r = make_double3(0, 0, 0);
@@ -70,6 +76,7 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
         }
     } else if ((idx -= P.n_bonds) < P.n_angles) {
         const Angle A = P.angles[idx];
+        if ((*live == 0) | (A.i < -1)) return;
         const Slots<TM> dst{P.dst_angle + 3 * idx, 2 * P.n_bonds + 3 * idx};
         double fm = 0.0;
         double3 di = make_double3(0, 0, 0), dk = make_double3(0, 0, 0);
@@ -97,6 +104,7 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
         }
     } else if ((idx -= P.n_angles) < P.n_dih) {
         const Dihedral Q = P.dih[idx];
+        if ((*live == 0) | (Q.i < -1)) return;
         const Slots<TM> dst{P.dst_dih + 4 * idx, 2 * P.n_bonds + 3 * P.n_angles + 4 * idx};
         double3 f1 = make_double3(0, 0, 0), f4 = make_double3(0, 0, 0), sv = make_double3(0, 0, 0);
         if ((mask & PRISM_MD_TERM_DIHEDRAL) && Q.i >= 0) {
@@ -131,6 +139,7 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
         }
     } else if ((idx -= P.n_dih) < P.n_exc) {
         const Exception14 X = P.exc[idx];
+        if ((*live == 0) | (X.i < -1)) return;
         const Slots<TM> dst{P.dst_exc + 2 * idx, 2 * P.n_bonds + 3 * P.n_angles + 4 * P.n_dih + 2 * idx};
         double f = 0.0;
         double3 d = make_double3(0, 0, 0);
