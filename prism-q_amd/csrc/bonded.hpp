@@ -53,6 +53,7 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
     // record's first index, never below -1, enters the test so that the
     // compiler cannot move the record load after it)
     const unsigned char *live = P.term_live + tid / TERM_CHUNK;
+    if (P.terms_sparse && !*live) return;  // a pool's free chunks skip their records
     int idx = tid;
     double energy = 0.0;
     if (idx < P.n_bonds) {

@@ -707,6 +707,7 @@ int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
 int Engine::init_pool(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     if (caps.empty()) return fail(PRISM_MD_E_INVALID, "no slots");
     if (int rc = layout(caps, stream)) return rc;
+    P_.terms_sparse = 1;
     if (int rc = upload_term_live()) return rc;  // no system yet: every chunk off
     return check_status();
 }

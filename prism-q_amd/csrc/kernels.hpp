@@ -47,6 +47,10 @@ struct DevPtrs {
     // bonded threads of other chunks return at once -- a serving pool's
     // arena is mostly free slots and padding.
     const unsigned char *term_live;
+    // 1 for a serving pool: most chunks are free there, so a bonded thread
+    // tests its chunk's flag before loading its record; elsewhere both load
+    // together (one wait instead of two for a live term)
+    int terms_sparse;
     unsigned *hs_group;             // [second-half workgroups / HS_GROUP] step-end handshake counters
     double *term_e;                 // per-term energy (bonds, angles, dihedrals, exceptions)
     double *atom_e;                 // per-atom nonbonded energy of the last gather
