@@ -104,35 +104,46 @@ def cpu_chain_c1(pm, n_atoms=2000, steps=1000):
                       f"{wall:.1f} s wall, {sps:.1f} steps/s, OpenMP threads={threads}"}
 
 
-def nve_leg(pm, n_atoms, steps, dt_ps, every):
+def nve_leg(pm, n_atoms, steps, dt_ps, every, seed=7, minimize=2000):
     """Energy drift without a thermostat (gamma = 0: exact kicks, no clamp):
-    the minimised chain from 300 K Maxwell-Boltzmann velocities, total energy
-    sampled every `every` steps; drift = least-squares slope."""
+    the minimised chain from 300 K Maxwell-Boltzmann velocities.  The first
+    step's RATTLE removes the bond-parallel part of those velocities (the
+    draw knows masses only), so the reference energy e0 is taken after that
+    first step; then the total energy is sampled every `every` steps and the
+    drift is the least-squares slope."""
     import numpy as np
-    top = pm.Topology.generate(n_atoms, seed=7)
+    top = pm.Topology.generate(n_atoms, seed=seed)
     p = pm.default_params(dt_ps=dt_ps, gamma_ps=0.0, equilibration_steps=0, com_remove_every=0)
     t0 = time.perf_counter()
     with pm.FusedEngine(top, p) as eng:
-        eng.minimize(2000, 1.0)
+        eng.minimize(minimize, 1.0)
         eng.set_state(v=pm.maxwell_boltzmann(top["masses"], 300.0, 3))
+        eng.step(1)  # constraint projection of the initial velocities
         ts, es, ks = [], [], []
+        fault = None
         for k in range(steps // every + 1):
             if k:
-                eng.step(every)
+                try:
+                    eng.step(every)
+                except pm.PrismMdError as ex:  # a latched fault ends the series, reported
+                    fault = {"after_steps": (k - 1) * every, "error": str(ex)}
+                    break
             en = eng.energies()
             ts.append(k * every * dt_ps)
             es.append(en["potential"] + en["kinetic"])
             ks.append(en["kinetic"])
+        st = eng.stats()
     t = np.asarray(ts)
     e = np.asarray(es)
     slope = float(np.polyfit(t, e, 1)[0])
     return {"atoms": n_atoms, "steps": steps, "dt_ps": dt_ps, "sample_every": every,
+            "baseline": "after the first step (velocities projected onto the constraints)",
             "drift_kcal_mol_per_ps": round(slope, 4),
             "drift_kcal_mol_per_ns_per_atom": round(slope * 1e3 / n_atoms, 5),
             "total_energy_change_kcal_mol": round(float(e[-1] - e[0]), 3),
             "mean_kinetic_kcal_mol": round(float(np.mean(ks)), 3),
             "max_abs_deviation_kcal_mol": round(float(np.max(np.abs(e - e[0]))), 3),
-            "wall_s": round(time.perf_counter() - t0, 1)}
+            "status": st["status"], "fault": fault, "wall_s": round(time.perf_counter() - t0, 1)}
 
 
 def large_leg(pm, n_atoms, steps, warmup, sample_every, chunk=1000):
@@ -440,6 +451,15 @@ def main():
     eng.step(args.steps)  # returns after the engine's stream is synchronised
     wall = time.perf_counter() - t0
     nb_ms, nb_samples = eng.kernel_timing(0)
+    timed_samples = nb_samples
+    if nb_samples < 100:
+        # a short timed region holds few sampled launches: extend the sample
+        # with the launches of the steps right after it, every step sampled
+        eng.kernel_timing(1)
+        eng.step(200)
+        ext_ms, ext_n = eng.kernel_timing(0)
+        nb_ms = (nb_ms * nb_samples + ext_ms * ext_n) / (nb_samples + ext_n)
+        nb_samples += ext_n
     st = eng.stats()
     pairs = eng.count_pairs()
     en = eng.energies()
@@ -463,10 +483,11 @@ def main():
         "pair_interactions_per_s": round(pairs * sps, 1),
         "unique_pairs_in_cutoff": pairs,
         "pair_rebuilds": st["pair_rebuilds"],
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": traffic,
                      "compute": "FP32 VALU (no MFMA: the pair term is not a contraction)",
                      "kernel": "k_nonbonded", "mean_launch_ms": round(nb_ms, 5), "samples": nb_samples,
+                     "samples_in_timed_region": timed_samples,
                      "flop_per_launch": FLOP_PER_PAIR * pairs},
         "cpu_baseline": None,
     }
@@ -484,6 +505,9 @@ def main():
     if args.nve_steps > 0:
         out["nve"] = [nve_leg(pm, 2000, args.nve_steps, dt, max(1, args.nve_steps // 20))
                       for dt in (args.nve_dt, 0.5 * args.nve_dt)]
+        # the production case: the 25,000-atom chain at 4 fs with HMR, no thermostat
+        out["nve"].append(nve_leg(pm, args.atoms, args.nve_steps, 0.004, max(1, args.nve_steps // 20),
+                                  seed=GEN_SEED))
         log(f"nve: {out['nve']}")
     out["serving"] = None
     if args.serving_seconds > 0:

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define PRISM_MD_ABI_VERSION 1
+#define PRISM_MD_ABI_VERSION 2  /* 2: energies out[2] is the work ledger W */
 
 /* ---- return codes ------------------------------------------------------ */
 #define PRISM_MD_OK               0
@@ -46,6 +46,8 @@ extern "C" {
 #define PRISM_MD_E_NOMEM         -6  /* device allocation failed at create    */
 #define PRISM_MD_E_UNSUPPORTED   -7  /* feature not available in this build   */
 #define PRISM_MD_E_NODEVICE      -8  /* no HIP device visible                 */
+#define PRISM_MD_E_DIVERGED      -9  /* a coordinate beyond 1000 A (heartbeat_check,
+                                        housekeeping.cu:27-45)                */
 
 /* ---- force-term mask (prism_md_compute_forces) -------------------------- */
 #define PRISM_MD_TERM_BOND       0x01u /* compute_bond_force        nhs_amber_fused.cu:354-386 */
@@ -259,12 +261,26 @@ int prism_md_compute_forces_fixed(PrismMdHandle *h, unsigned term_mask,
  * :23911, and the captured replay of persistent_engine.rs:3207-3300. */
 int prism_md_step(PrismMdHandle *h, uint64_t n_steps, const float *noise);
 
-/* out[0] = potential energy (sum of all terms, kcal/mol), out[1] = kinetic
- * energy (kcal/mol), out[2] = kinetic temperature (K) over 3N - N_constraints - 3
- * degrees of freedom; sums reduced deterministically in FP64 / fixed point.
- * Replaces prism_energy_monitor_* (energy_monitor.cu:24-99) and the
- * nhs_diagnostic_phase_energy_v1 sums (nhs_amber_fused.cu:14840-14934). */
+/* out[0] = potential energy PE (sum of all terms, kcal/mol), out[1] = kinetic
+ * energy KE (kcal/mol), out[2] = W, the cumulative non-Hamiltonian work
+ * (kcal/mol) since the system was created or admitted: the kinetic-energy
+ * change of every Ornstein-Uhlenbeck half-step, of the RATTLE projections that
+ * follow a bath update (after OU_N, and the final one), of the 1000 kcal/mol/A
+ * kick clamp and of the velocity safety clamp, exactly the entries the
+ * reference books into d_external_work per step (nhs_amber_fused.cu:1713-1788,
+ * :2245-2270, :11148-11186; routing fused_engine.rs:19273-19277, :17698-17740)
+ * and reduces with nhs_reduce_external_work_components (:11097-11126).  The
+ * pre-drift RATTLE, SHAKE and pre-OU_NP1 RATTLE go to the reference's separate
+ * constraint planes and are not in W.  Sums reduced deterministically in FP64.
+ * Replaces prism_energy_monitor_* (energy_monitor.cu:24-99), the
+ * nhs_diagnostic_phase_energy_v1 sums (nhs_amber_fused.cu:14840-14934) and the
+ * W_ext window read (fused_engine.rs:10700-10708).  ABI version 1 returned the
+ * kinetic temperature in out[2]; it is now prism_md_kinetic_temperature. */
 int prism_md_energies(PrismMdHandle *h, double out[3]);
+
+/* Kinetic temperature (K) over 3N - N_constraints - 3 degrees of freedom
+ * (the reference's temperature diagnostic, nhs_amber_fused.cu:14840-14934). */
+int prism_md_kinetic_temperature(PrismMdHandle *h, double *T);
 
 int prism_md_get_stats(PrismMdHandle *h, PrismMdStatsV1 *stats);
 
@@ -324,6 +340,7 @@ int prism_md_batch_get_state(PrismMdBatch *b, int32_t system, float *x, float *v
 int prism_md_batch_compute_forces(PrismMdBatch *b, int32_t system, unsigned term_mask, float *F_out,
                                   double *E_terms_out);
 int prism_md_batch_energies(PrismMdBatch *b, int32_t system, double out[3]);
+int prism_md_batch_kinetic_temperature(PrismMdBatch *b, int32_t system, double *T);
 int prism_md_batch_get_stats(PrismMdBatch *b, int32_t system, PrismMdStatsV1 *stats);
 const char *prism_md_batch_last_error(PrismMdBatch *b);
 

@@ -24,6 +24,10 @@
 //                  (fused_engine.rs:4723-4731), X-H lengths after SHAKE, zero
 //                  bond-parallel relative velocity after RATTLE, momentum
 //                  conservation of SHAKE, second-order NVE energy error.
+//   oracle_run work the energy balance: E(t) - E(0) - W is a second-order
+//                  shadow error while W is hundreds of kcal/mol, so a dropped,
+//                  extra or sign-flipped ledger entry fails; W == 0 without a
+//                  bath or clamps.
 //   oracle_philox  Random123 known-answer vectors quoted at
 //                  scripts/physics_refs/rng_streams.py:82-88, and the keyed
 //                  counter layout of rng_streams.py:75-79.
@@ -566,9 +570,28 @@ void half_kick(int n, const float *mass, double dt, double gamma, const double *
     }
 }
 
-int rattle_all(const PrismMdTopologyV1 *t, const double *x, double *v, const PrismMdParamsV1 *p) {
-    for (int c = 0; c < t->n_clusters; ++c)
+// RATTLE over every cluster.  With work != NULL the projection's
+// kinetic-energy change is booked with the cluster's masses 1/inv_mass
+// (rattle_cluster_velocities, nhs_amber_fused.cu:2245-2270): the reference
+// books the RATTLE after OU_N and the final one into its work ledger and
+// routes the pre-drift and pre-OU_NP1 ones to separate constraint planes
+// (fused_engine.rs:19273-19277, :17735-17740).
+int rattle_all(const PrismMdTopologyV1 *t, const double *x, double *v, const PrismMdParamsV1 *p, double *work) {
+    for (int c = 0; c < t->n_clusters; ++c) {
+        const int a = t->cluster_central[c];
+        const double mc = 1.0 / (double)t->cluster_inv_mass_central[c], mh = 1.0 / (double)t->cluster_inv_mass_h[c];
+        auto ke = [&]() {
+            double k = 0.5 * mc * vdot({v[3 * a], v[3 * a + 1], v[3 * a + 2]}, {v[3 * a], v[3 * a + 1], v[3 * a + 2]});
+            for (int h = 0; h < t->cluster_n_hydrogens[c]; ++h) {
+                const int b = t->cluster_hydrogens[3 * c + h];
+                if (b >= 0) k += 0.5 * mh * (v[3 * b] * v[3 * b] + v[3 * b + 1] * v[3 * b + 1] + v[3 * b + 2] * v[3 * b + 2]);
+            }
+            return k / kKcalToAmu;
+        };
+        const double before = work ? ke() : 0.0;
         if (oracle_rattle_cluster(t, c, x, v, p->rattle_tol, p->constraint_max_iter) < 0) return PRISM_MD_E_CONSTRAINT;
+        if (work) *work += ke() - before;
+    }
     return PRISM_MD_OK;
 }
 
@@ -623,9 +646,9 @@ extern "C" int oracle_run(const PrismMdTopologyV1 *t, const PrismMdParamsV1 *p, 
             draw(k, s, 0);
             ou_half(n, t->masses, T, gamma, dt, v, xi.data(), work);
         }
-        if ((rc = rattle_all(t, x, v, p))) return rc;
+        if ((rc = rattle_all(t, x, v, p, work))) return rc;  // RATTLE_AFTER_OU_N: booked
         half_kick(n, t->masses, dt, gamma, F.data(), v, work);
-        if ((rc = rattle_all(t, x, v, p))) return rc;
+        if ((rc = rattle_all(t, x, v, p, nullptr))) return rc;  // RATTLE_PRE_DRIFT
         for (size_t q = 0; q < xs.size(); ++q) {
             xs[q] = x[q];
             x[q] += dt * v[q];
@@ -635,7 +658,7 @@ extern "C" int oracle_run(const PrismMdTopologyV1 *t, const PrismMdParamsV1 *p, 
                 return PRISM_MD_E_CONSTRAINT;
         if ((rc = force())) return rc;
         half_kick(n, t->masses, dt, gamma, F.data(), v, work);
-        if ((rc = rattle_all(t, x, v, p))) return rc;
+        if ((rc = rattle_all(t, x, v, p, nullptr))) return rc;  // RATTLE_BEFORE_OU_NP1
         if (gamma != 0.0) {
             draw(k, s, 1);
             ou_half(n, t->masses, T, gamma, dt, v, xi.data(), work);
@@ -650,7 +673,7 @@ extern "C" int oracle_run(const PrismMdTopologyV1 *t, const PrismMdParamsV1 *p, 
                 if (work) *work += ke_atom(m, v + 3 * i) - before;
             }
         }
-        if ((rc = rattle_all(t, x, v, p))) return rc;
+        if ((rc = rattle_all(t, x, v, p, work))) return rc;  // final RATTLE: booked
         for (size_t q = 0; q < xs.size(); ++q)
             if (!std::isfinite(x[q]) || !std::isfinite(v[q])) return PRISM_MD_E_NONFINITE;
     }

@@ -39,7 +39,9 @@ int64_t oracle_pairs(const PrismMdTopologyV1 *t, const float *x, const float *x_
 /* n_steps of constrained Langevin OVRVO from step index step0 (0-based, the
  * reference's timestep counter).  x, v: [3n] double, updated in place.
  * noise: [n_steps][2][n][3] standard normals or NULL for Philox noise.
- * work: accumulated non-Hamiltonian work (thermostat and clamps), may be NULL.
+ * work: accumulated non-Hamiltonian work, may be NULL: the kinetic-energy
+ * change of the OU half-steps, the RATTLE after OU_N and the final RATTLE,
+ * the force clamp and the velocity clamp (the reference's d_external_work).
  * Returns 0, PRISM_MD_E_CONSTRAINT or PRISM_MD_E_NONFINITE. */
 int oracle_run(const PrismMdTopologyV1 *t, const PrismMdParamsV1 *p, double *x, double *v, int64_t step0,
                int64_t n_steps, const float *noise, double *work);

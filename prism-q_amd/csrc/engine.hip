@@ -315,11 +315,12 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     P_.vals_a = dalloc<int>(A);
     P_.vals_b = dalloc<int>(A);
     P_.com_partial = dalloc<double>((size_t)W * 3);
+    P_.work_g = dalloc<double>((size_t)W * 64);
     P_.sweeps = dalloc<int>((size_t)W * 2);
     P_.counters = dalloc<EngineCounters>(1);
     P_.hs_group = dalloc<unsigned>((size_t)W / 64 + 2);  // >= second-half workgroups / 128, zeroed
     d_count_ = dalloc<unsigned long long>(1);
-    d_red_ = dalloc<double>((size_t)n_sys * 4);
+    d_red_ = dalloc<double>((size_t)n_sys * RED_STRIDE);
     d_noise_ = dalloc<float>((size_t)2 * A * 3);
     d_vmin_ = dalloc<float4>(A);
     d_fire_ = dalloc<double>((size_t)(A / 256 + 1) * 4);
@@ -682,6 +683,11 @@ int Engine::upload_slot(int s) {
                                       hipMemcpyHostToDevice, stream_),
                        "slot sweeps")))
         return rc;
+    // the work ledger starts at zero with the system
+    const std::vector<double> zw(ng, 0.0);
+    if ((rc = hipcheck(hipMemcpyAsync(P_.work_g + g0, zw.data(), ng * sizeof(double), hipMemcpyHostToDevice, stream_),
+                       "slot work")))
+        return rc;
     // the host staging buffers above must outlive the copies
     return hipcheck(hipStreamSynchronize(stream_), "slot upload");
 }
@@ -807,8 +813,8 @@ int Engine::check_status() {
         // get_stats but no longer fails calls on the systems still running
         if (!b || !desc_[s].active) continue;
         static const char *kinds[] = {"?", "RATTLE after OU_N", "RATTLE before drift", "SHAKE", "RATTLE before OU_NP1",
-                                      "final RATTLE", "non-finite state"};
-        const int fk = st[s].fault_kind >= 0 && st[s].fault_kind <= 6 ? st[s].fault_kind : 0;
+                                      "final RATTLE", "non-finite state", "coordinate beyond 1000 A"};
+        const int fk = st[s].fault_kind >= 0 && st[s].fault_kind <= 7 ? st[s].fault_kind : 0;
         const std::string where = " (system " + std::to_string(s) + ", " + kinds[fk] + ", atom " +
                                   std::to_string(st[s].fault_atom - sys_[s].atom_begin) + ", step " +
                                   std::to_string(st[s].fault_step) + ")";
@@ -817,6 +823,7 @@ int Engine::check_status() {
         if ((b & ST_CONSTRAINT) && fk >= FAULT_RATTLE_1 && fk <= FAULT_RATTLE_4)
             return fail(PRISM_MD_E_CONSTRAINT, "constraint iteration did not converge" + where);
         if (b & ST_NONFINITE) return fail(PRISM_MD_E_NONFINITE, "non-finite state" + where);
+        if (b & ST_DIVERGED) return fail(PRISM_MD_E_DIVERGED, "system diverged: a coordinate beyond 1000 A" + where);
         if (b & ST_CONSTRAINT) return fail(PRISM_MD_E_CONSTRAINT, "constraint iteration did not converge" + where);
         if (b & ST_EXCEPTION_COINCIDENT) return fail(PRISM_MD_E_NONFINITE, "coincident exception atoms" + where);
         if (b & ST_ROW_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "tile row overflow" + where);
@@ -1010,13 +1017,30 @@ int Engine::step_graph(uint64_t n_steps, int graph_steps) {
 int Engine::energies(int s, double out[3]) {
     double E[PRISM_MD_N_TERMS];
     if (int rc = compute_forces(PRISM_MD_TERM_ALL, s, nullptr, nullptr, E)) return rc;
-    launch_kinetic(P_, d_red_, -1, stream_);
-    double red[4];
-    if (int rc = hipcheck(hipMemcpyAsync(red, d_red_ + 4 * s, sizeof(red), hipMemcpyDeviceToHost, stream_), "ke copy")) return rc;
-    if (int rc = hipcheck(hipStreamSynchronize(stream_), "sync")) return rc;
+    double red[RED_STRIDE];
+    if (int rc = reduce_system(s, red)) return rc;
     out[0] = E[0] + E[1] + E[2] + E[3] + E[4];
     out[1] = red[0];
-    out[2] = 2.0 * red[0] / (sys_[s].dof * KB);
+    out[2] = red[4];
+    return PRISM_MD_OK;
+}
+
+// Kinetic energy, COM velocity and cumulative work of system s (FP64, fixed
+// order), without touching the carried COM velocity.
+int Engine::reduce_system(int s, double red[RED_STRIDE]) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    launch_kinetic(P_, d_red_, -1, stream_);
+    if (int rc = hipcheck(hipMemcpyAsync(red, d_red_ + (size_t)RED_STRIDE * s, RED_STRIDE * sizeof(double),
+                                         hipMemcpyDeviceToHost, stream_),
+                          "reduce copy"))
+        return rc;
+    return hipcheck(hipStreamSynchronize(stream_), "sync");
+}
+
+int Engine::temperature(int s, double *T) {
+    double red[RED_STRIDE];
+    if (int rc = reduce_system(s, red)) return rc;
+    *T = 2.0 * red[0] / (sys_[s].dof * KB);
     return PRISM_MD_OK;
 }
 
@@ -1241,11 +1265,13 @@ int Engine::set_active(int s, bool active) {
 // step), so the step counter is the whole RNG state.
 namespace {
 constexpr char CKPT_MAGIC[8] = {'P', 'R', 'I', 'S', 'M', 'C', 'K', '1'};
+constexpr uint32_t CKPT_VERSION = 2;  // 2: the cumulative work ledger
 struct CheckpointHeader {
     char magic[8];
     uint32_t version, header_bytes;
     int32_t n_atoms, n_blocks;
     uint64_t topo_digest, params_digest, payload_bytes;
+    double work;                    // cumulative non-Hamiltonian work, kcal/mol
     SysState state;
 };
 size_t payload_bytes(int n, int nb) {
@@ -1268,7 +1294,7 @@ int Engine::checkpoint_save(int s, void *buf, size_t cap, size_t *written) {
     if (int rc = hipcheck(hipStreamSynchronize(stream_), "checkpoint sync")) return rc;
     CheckpointHeader hd{};
     std::memcpy(hd.magic, CKPT_MAGIC, 8);
-    hd.version = 1;
+    hd.version = CKPT_VERSION;
     hd.header_bytes = sizeof(CheckpointHeader);
     hd.n_atoms = n;
     hd.n_blocks = nb;
@@ -1276,6 +1302,11 @@ int Engine::checkpoint_save(int s, void *buf, size_t cap, size_t *written) {
     hd.params_digest = H.params_digest;
     hd.payload_bytes = payload_bytes(n, nb);
     if (int rc = d2h(&hd.state, P_.sst + s, sizeof(SysState))) return rc;
+    {
+        double red[RED_STRIDE];
+        if (int rc = reduce_system(s, red)) return rc;
+        hd.work = red[4];
+    }
     char *out = static_cast<char *>(buf);
     std::memcpy(out, &hd, sizeof hd);
     char *p = out + sizeof hd;
@@ -1304,7 +1335,7 @@ int Engine::checkpoint_restore(int s, const void *buf, size_t len) {
     CheckpointHeader hd;
     if (len < sizeof hd) return fail(PRISM_MD_E_INVALID, "restore: buffer shorter than a checkpoint header");
     std::memcpy(&hd, buf, sizeof hd);
-    if (std::memcmp(hd.magic, CKPT_MAGIC, 8) != 0 || hd.version != 1 || hd.header_bytes != sizeof hd)
+    if (std::memcmp(hd.magic, CKPT_MAGIC, 8) != 0 || hd.version != CKPT_VERSION || hd.header_bytes != sizeof hd)
         return fail(PRISM_MD_E_INVALID, "restore: not a prism_md checkpoint of this version");
     if (hd.n_atoms != n || hd.n_blocks != nb || hd.topo_digest != H.topo_digest)
         return fail(PRISM_MD_E_INVALID, "restore: checkpoint of a different topology");
@@ -1337,6 +1368,13 @@ int Engine::checkpoint_restore(int s, const void *buf, size_t len) {
         (rc = hipcheck(hipMemcpyAsync(P_.perm + (size_t)H.block_begin * BLOCK, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "restore perm")) ||
         (rc = hipcheck(hipMemcpyAsync(P_.bb_c + H.block_begin, boxes.data(), (size_t)nb * sizeof(float4), hipMemcpyHostToDevice, stream_), "restore boxes")) ||
         (rc = hipcheck(hipMemcpyAsync(P_.bb_h + H.block_begin, boxes.data() + nb, (size_t)nb * sizeof(float4), hipMemcpyHostToDevice, stream_), "restore boxes")))
+        return rc;
+    // the work ledger: the saved total in the system's first group, zeros elsewhere
+    std::vector<double> wg((size_t)H.n_waves * 64, 0.0);
+    wg[0] = hd.work;
+    if ((rc = hipcheck(hipMemcpyAsync(P_.work_g + (size_t)H.wave_begin * 64, wg.data(), wg.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, stream_),
+                       "restore work")))
         return rc;
     launch_restore_slots(P_, s, nb * BLOCK, stream_);
     // rows and exclusion masks of the restored blocks from the restored boxes
@@ -1447,6 +1485,9 @@ int prism_md_step(PrismMdHandle *h, uint64_t n_steps, const float *noise) {
     return h ? h->eng.step(n_steps, noise) : PRISM_MD_E_INVALID;
 }
 int prism_md_energies(PrismMdHandle *h, double out[3]) { return h && out ? h->eng.energies(0, out) : PRISM_MD_E_INVALID; }
+int prism_md_kinetic_temperature(PrismMdHandle *h, double *T) {
+    return h && T ? h->eng.temperature(0, T) : PRISM_MD_E_INVALID;
+}
 int prism_md_get_stats(PrismMdHandle *h, PrismMdStatsV1 *stats) { return h ? h->eng.stats(0, stats) : PRISM_MD_E_INVALID; }
 const char *prism_md_last_error(PrismMdHandle *h) { return h ? h->eng.last_error() : g_create_error.c_str(); }
 int prism_md_count_pairs(PrismMdHandle *h, uint64_t *n) { return h && n ? h->eng.count_pairs(0, n) : PRISM_MD_E_INVALID; }
@@ -1525,6 +1566,9 @@ int prism_md_batch_compute_forces(PrismMdBatch *b, int32_t system, unsigned term
     return b ? b->eng.compute_forces(term_mask, system, F_out, nullptr, E_terms_out) : PRISM_MD_E_INVALID;
 }
 
+int prism_md_batch_kinetic_temperature(PrismMdBatch *b, int32_t system, double *T) {
+    return b && T ? b->eng.temperature(system, T) : PRISM_MD_E_INVALID;
+}
 int prism_md_batch_energies(PrismMdBatch *b, int32_t system, double out[3]) {
     return b && out ? b->eng.energies(system, out) : PRISM_MD_E_INVALID;
 }

@@ -47,7 +47,8 @@ class Engine {
     int get_state(int s, float *x, float *v, int flags);
     int compute_forces(unsigned mask, int s, float *F_out, int64_t *F_fixed, double *E_terms);
     int step(uint64_t n_steps, const float *noise);  // noise: single-system explicit array or NULL
-    int energies(int s, double out[3]);
+    int energies(int s, double out[3]);   // potential, kinetic, cumulative work
+    int temperature(int s, double *T);    // kinetic temperature over the system's degrees of freedom
     int build_pairs(int s, int mode);
     int get_pairs(int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out);
     int count_pairs(int s, uint64_t *n);
@@ -78,6 +79,7 @@ class Engine {
     int fail(int rc, const std::string &msg);
     int hipcheck(hipError_t e, const char *what);
     int d2h(void *dst, const void *src, size_t bytes);  // stream-ordered copy + synchronise
+    int reduce_system(int s, double red[RED_STRIDE]);
     int refresh_all();            // rebuild lists + forces at current positions
     int refresh(int only);        // the same for one system (-1: all)
     int layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream);

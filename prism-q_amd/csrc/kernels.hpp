@@ -79,6 +79,8 @@ struct DevPtrs {
     unsigned int *keys_a, *keys_b;
     int *vals_a, *vals_b;
     double *com_partial;            // [waves][3]
+    double *work_g;                 // [waves * 64] per constraint group: cumulative non-Hamiltonian
+                                    // work (bath, clamps), written only by the group's own lane
     int *sweeps;                    // [waves][2]: most SHAKE / RATTLE sweeps so far, per group wave
                                     // (written only by that wave; reduced on the host)
     EngineCounters *counters;
@@ -115,8 +117,9 @@ void launch_zero_forces(const DevPtrs &P, hipStream_t s);
 void launch_gather(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 // Per-system FP64 energy sums of the last evaluation into SysState.energy.
 void launch_energy_reduce(const DevPtrs &P, hipStream_t s);
-// ke_out[S][4] = (KE, COM velocity); com_sys: system whose carried COM
-// velocity is also set (-2 all, -1 none).
+// ke_out[S][RED_STRIDE] = (KE, COM velocity, cumulative work); com_sys: system
+// whose carried COM velocity is also set (-2 all, -1 none).
+constexpr int RED_STRIDE = 5;
 void launch_kinetic(const DevPtrs &P, double *ke_out, int com_sys, hipStream_t s);
 void launch_emit_pairs(const DevPtrs &P, int mode, RefGrid g, int *out_i, int *out_j, unsigned long long *count,
                        unsigned long long cap, hipStream_t s);

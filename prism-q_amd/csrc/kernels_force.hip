@@ -340,9 +340,16 @@ __global__ void k_zero_forces(DevPtrs P) {
     if (i < (size_t)P.n_atoms_arena * 4) P.f[i] = 0;
 }
 
-__device__ inline float r2_list(float3 a, float3 b) {  // (dx*dx + dy*dy) + dz*dz, dx = b - a, no contraction
-    const float dx = __fsub_rn(b.x, a.x), dy = __fsub_rn(b.y, a.y), dz = __fsub_rn(b.z, a.z);
-    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+// (dx*dx + dy*dy) + dz*dz with dx = b - a, every operation rounded on its own.
+// HIP's __fadd_rn / __fmul_rn are the plain operators here, and hipcc
+// contracts a*b + c into an FMA by default, which rounds once instead of
+// twice: a pair at FP32 r2 = 144.0 exactly then tested as > 144 and left the
+// list (found at 250,000 atoms).  Contraction is switched off for this
+// function, so the test rounds as the reference's list does.
+__device__ inline float r2_list(float3 a, float3 b) {
+#pragma clang fp contract(off)
+    const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    return (dx * dx + dy * dy) + dz * dz;
 }
 __device__ inline bool in_list(const int *off, const int *idx, int a, int b) {
     for (int e = off[a]; e < off[a + 1]; ++e)
@@ -350,7 +357,7 @@ __device__ inline bool in_list(const int *off, const int *idx, int a, int b) {
     return false;
 }
 __device__ inline int ref_cell(float x, float o, int n) {  // nhs_amber_fused.cu:11704-11712
-    const int c = (int)__fmul_rn(__fsub_rn(x, o), 0.1f);
+    const int c = (int)((x - o) * 0.1f);  // a subtract then a multiply: nothing to contract
     return max(0, min(c, n - 1));
 }
 

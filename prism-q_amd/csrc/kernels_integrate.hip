@@ -31,6 +31,7 @@ struct GroupState {
     float m[4];
     float x[4][3], v[4][3];
     long long f[4][3];  // the carried force, raw int64 fixed point
+    double w;           // the group's cumulative non-Hamiltonian work, kcal/mol
 };
 
 // What the first half reads at its end: each atom's sorted slot and its
@@ -221,7 +222,19 @@ __device__ int shake(GroupState &g, const float xs[4][3], const GroupParam &gp, 
     return used;
 }
 
-// langevin_thermostat with dt/2 (nhs_amber_fused.cu:1713-1744, :3656-3666).
+// |v|^2 in FP64 of an FP32 velocity (the work ledger's kinetic energies).
+__device__ inline double speed2(const float v[3]) {
+    return (double)v[0] * (double)v[0] + (double)v[1] * (double)v[1] + (double)v[2] * (double)v[2];
+}
+// 0.5 m (|v_after|^2 - |v_before|^2) / 418.4, the kinetic-energy change every
+// entry of the reference's work ledger books (nhs_amber_fused.cu:1734-1743,
+// :1776-1785, :11178-11185).
+__device__ inline double ke_change(double m, double before2, double after2) {
+    return 0.5 * m * (after2 - before2) * (1.0 / KCAL_TO_AMU_A2_PS2);
+}
+
+// langevin_thermostat with dt/2 (nhs_amber_fused.cu:1713-1744, :3656-3666),
+// booking each atom's kinetic-energy change in the work ledger.
 __device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float gamma, unsigned long long step,
                                int purpose, const float *noise) {
     const float c1 = expf(-gamma * (0.5f * D.dt));
@@ -236,7 +249,9 @@ __device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float g
         // forms cost ~10 dependent instructions each on a latency-bound lane
         const float c2 = __builtin_amdgcn_sqrtf((1.0f - c1 * c1) * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T *
                                                 __builtin_amdgcn_rcpf(g.m[k]));
+        const double before = speed2(g.v[k]);
         for (int c = 0; c < 3; ++c) g.v[k][c] = c1 * g.v[k][c] + c2 * xi[c];
+        g.w += ke_change(g.m[k], before, speed2(g.v[k]));
     }
 }
 
@@ -251,16 +266,35 @@ __device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, floa
         for (int c = 0; c < 3; ++c) f[c] = from_fixed(g.f[k][c], FORCE_FIXED_BITS);
         if (consume)
             for (int c = 0; c < 3; ++c) P.f[(size_t)g.a[k] * 4 + c] = 0;
+        const float kick = 0.5f * dt * (float)KCAL_TO_AMU_A2_PS2 * __builtin_amdgcn_rcpf(g.m[k]);
         if (gamma != 0.0f) {  // |f| > MAX_FORCE tested squared; the rescale is rare
             const float mag2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
             if (mag2 > MAX_FORCE * MAX_FORCE) {
+                // the clamp's work: used minus unclamped candidate velocity
+                // (apply_force_half_kick_with_intervention_work, :1750-1788)
+                float raw[3];
+                for (int c = 0; c < 3; ++c) raw[c] = g.v[k][c] + kick * f[c];
                 const float q = MAX_FORCE * __builtin_amdgcn_rsqf(mag2);
                 for (int c = 0; c < 3; ++c) f[c] *= q;
+                for (int c = 0; c < 3; ++c) g.v[k][c] += kick * f[c];
+                g.w += ke_change(g.m[k], speed2(raw), speed2(g.v[k]));
+                continue;
             }
         }
-        const float kick = 0.5f * dt * (float)KCAL_TO_AMU_A2_PS2 * __builtin_amdgcn_rcpf(g.m[k]);
         for (int c = 0; c < 3; ++c) g.v[k][c] += kick * f[c];
     }
+}
+
+// Kinetic energy change of a RATTLE projection booked to the work ledger
+// with the cluster's masses (rattle_cluster_velocities, :2245-2270): the
+// projections that follow a bath update -- after OU_N and the final one after
+// OU_NP1 and the velocity clamp -- book there (fused_engine.rs:19273-19277,
+// :17735-17740); the pre-drift, SHAKE and pre-OU_NP1 solves go to the
+// reference's separate constraint planes and are not part of W.
+__device__ inline double group_ke2(const GroupState &g, const GroupParam &gp) {
+    double s = 0.0;
+    FOR_ATOMS(k) s += speed2(g.v[k]) / (double)(k == 0 ? gp.inv_mass_c : gp.inv_mass_h);
+    return s;
 }
 
 // Load a constraint group and its atoms' state in two rounds of memory
@@ -293,16 +327,27 @@ __device__ inline bool load_group(const DevPtrs &P, int gidx, GroupState &g, Gro
             t.xb[k][0] = b.x; t.xb[k][1] = b.y; t.xb[k][2] = b.z;
         }
     }
+    g.w = P.work_g[gidx];
     if (G.atom[0] < 0) return false;
     g.n = 1 + gp.n_h;
     return true;
+}
+
+// RATTLE whose kinetic-energy change goes to the work ledger (the bath-side
+// projections, see group_ke2).
+__device__ inline int rattle_booked(GroupState &g, const GroupParam &gp, double tol, int max_iter) {
+    if (g.n < 2) return 0;
+    const double before = group_ke2(g, gp);
+    const int r = rattle(g, gp, tol, max_iter);
+    g.w += 0.5 * (group_ke2(g, gp) - before) * (1.0 / KCAL_TO_AMU_A2_PS2);
+    return r;
 }
 
 __device__ inline bool finite3(const float a[3]) { return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]); }
 
 // Latch the first failure of a system: status bit plus kind, atom and step.
 __device__ inline void record_fault(SysState &S, int kind, int atom, unsigned long long step) {
-    atomicOr(&S.status, kind == FAULT_NONFINITE ? ST_NONFINITE : ST_CONSTRAINT);
+    atomicOr(&S.status, kind == FAULT_NONFINITE ? ST_NONFINITE : (kind == FAULT_DIVERGED ? ST_DIVERGED : ST_CONSTRAINT));
     if (atomicCAS(&S.fault_atom, -1, atom) == -1) {
         S.fault_kind = kind;
         S.fault_step = step;
@@ -353,7 +398,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
             }
         }
         if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 1, noise);
-        note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_1, g.a[0], step, most_ra);
+        note_sweeps(S, rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_1, g.a[0], step, most_ra);
         half_kick(g, P, D.dt, gamma, true);
         note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_2, g.a[0], step, most_ra);
         float xs[4][3];
@@ -367,6 +412,10 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
         FOR_ATOMS(k) {
             const int a = g.a[k];
             if (!(finite3(g.x[k]) && finite3(g.v[k]))) record_fault(S, FAULT_NONFINITE, a, step);
+            // heartbeat_check (housekeeping.cu:27-45): a coordinate beyond
+            // 1000 A means the system has diverged
+            else if (fmaxf(fmaxf(fabsf(g.x[k][0]), fabsf(g.x[k][1])), fabsf(g.x[k][2])) > MAX_COORD)
+                record_fault(S, FAULT_DIVERGED, a, step);
             P.x[a] = make_float4(g.x[k][0], g.x[k][1], g.x[k][2], 0.0f);
             P.v[a] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
             float *q = (float *)&P.xq[tail.slot[k]];
@@ -374,6 +423,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
             const float dx = g.x[k][0] - tail.xb[k][0], dy = g.x[k][1] - tail.xb[k][1], dz = g.x[k][2] - tail.xb[k][2];
             moved |= dx * dx + dy * dy + dz * dz > D.half_skin2;
         }
+        P.work_g[gw * 64 + lane] = g.w;
     }
     publish_sweeps(P.sweeps + 2 * gw, prev_sh, most_sh);
     publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
@@ -417,11 +467,13 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
                                               __builtin_amdgcn_rcpf(fmaxf(g.m[k], 1.0f)), 4.0f);
                 const float sp2 = g.v[k][0] * g.v[k][0] + g.v[k][1] * g.v[k][1] + g.v[k][2] * g.v[k][2];
                 if (sp2 > vmax2) {
+                    const double before = speed2(g.v[k]);
                     const float sc = __builtin_amdgcn_sqrtf(vmax2) * __builtin_amdgcn_rsqf(sp2);
                     for (int c = 0; c < 3; ++c) g.v[k][c] *= sc;
+                    g.w += ke_change(g.m[k], before, speed2(g.v[k]));
                 }
             }
-            sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            sw = rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter);
             publish |= sw < 0;
             note_sweeps(S, sw, FAULT_RATTLE_4, g.a[0], step, most_ra);
             FOR_ATOMS(k) {
@@ -432,6 +484,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
                 P.v[g.a[k]] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
                 for (int c = 0; c < 3; ++c) mv[c] += (double)g.m[k] * g.v[k][c];
             }
+            P.work_g[gw * 64 + lane] = g.w;
         }
         publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
         if (next_com) {
@@ -508,16 +561,18 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
 }
 
 // Per-system reductions in FP64 (one workgroup per system, fixed order):
-// kinetic energy and centre-of-mass velocity.
-// Per-system kinetic energy and COM velocity.  The COM velocity carried to the
-// next COM-removal step is overwritten only for com_sys (a system index, -2 for
-// all, -1 for none), so reading energies never perturbs a trajectory.
-__global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs P, double *out /*[S][4]*/, int com_sys) {
+// kinetic energy, centre-of-mass velocity and the cumulative work ledger (the
+// role of nhs_reduce_external_work_components, nhs_amber_fused.cu:11097-11126,
+// summed here over constraint groups instead of atoms).  The COM velocity
+// carried to the next COM-removal step is overwritten only for com_sys (a
+// system index, -2 for all, -1 for none), so reading energies never perturbs
+// a trajectory.
+__global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs P, double *out /*[S][RED_STRIDE]*/, int com_sys) {
     const int s = blockIdx.x;
     if (s >= P.n_sys) return;
     const SysDesc &D = P.sys[s];
-    __shared__ double red[4][4];
-    double ke = 0.0, p[3] = {0.0, 0.0, 0.0};
+    __shared__ double red[4][5];
+    double ke = 0.0, p[3] = {0.0, 0.0, 0.0}, w = 0.0;
     for (int i = threadIdx.x; i < D.n_atoms; i += blockDim.x) {
         const int a = D.atom_begin + i;
         const float4 v = P.v[a];
@@ -525,20 +580,23 @@ __global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs P, double *out /*[S]
         ke += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
         p[0] += m * v.x; p[1] += m * v.y; p[2] += m * v.z;
     }
+    for (int i = threadIdx.x; i < D.n_waves * 64; i += blockDim.x) w += P.work_g[(size_t)D.wave_begin * 64 + i];
     ke = wave_sum(ke);
     for (int c = 0; c < 3; ++c) p[c] = wave_sum(p[c]);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) { red[w][0] = ke; red[w][1] = p[0]; red[w][2] = p[1]; red[w][3] = p[2]; }
+    w = wave_sum(w);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[wv][0] = ke; red[wv][1] = p[0]; red[wv][2] = p[1]; red[wv][3] = p[2]; red[wv][4] = w; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double r[4] = {0, 0, 0, 0};
+        double r[5] = {0, 0, 0, 0, 0};
         for (int k = 0; k < 4; ++k)
-            for (int c = 0; c < 4; ++c) r[c] += red[k][c];
-        out[s * 4 + 0] = r[0] / KCAL_TO_AMU_A2_PS2;
+            for (int c = 0; c < 5; ++c) r[c] += red[k][c];
+        out[s * RED_STRIDE + 0] = r[0] / KCAL_TO_AMU_A2_PS2;
         for (int c = 0; c < 3; ++c) {
-            out[s * 4 + 1 + c] = r[1 + c] * D.inv_total_mass;
+            out[s * RED_STRIDE + 1 + c] = r[1 + c] * D.inv_total_mass;
             if (com_sys == -2 || com_sys == s) P.sst[s].com_v[c] = r[1 + c] * D.inv_total_mass;
         }
+        out[s * RED_STRIDE + 4] = r[4];
     }
 }
 

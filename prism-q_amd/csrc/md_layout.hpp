@@ -35,6 +35,7 @@ constexpr double COULOMB = 332.0636;         // nhs_amber_fused.cu:66
 constexpr double KCAL_TO_AMU_A2_PS2 = 418.4; // nhs_amber_fused.cu:69
 constexpr double KB = 0.001987204;           // nhs_amber_fused.cu:70
 constexpr float MAX_FORCE = 1000.0f;         // nhs_amber_fused.cu:71
+constexpr float MAX_COORD = 1000.0f;         // divergence bound of heartbeat_check, housekeeping.cu:41
 
 // status bits latched on the device (SysState.status)
 enum : int {
@@ -43,8 +44,9 @@ enum : int {
     ST_ROW_OVERFLOW = 4, // more than MAX_ROW tiles in a row
     ST_MASK_OVERFLOW = 8,
     ST_EXCEPTION_COINCIDENT = 16,
+    ST_DIVERGED = 32,    // a coordinate beyond MAX_COORD
     // any of these halts the system's integration at the end of the step
-    ST_HALT = ST_CONSTRAINT | ST_NONFINITE | ST_ROW_OVERFLOW | ST_MASK_OVERFLOW | ST_EXCEPTION_COINCIDENT,
+    ST_HALT = ST_CONSTRAINT | ST_NONFINITE | ST_ROW_OVERFLOW | ST_MASK_OVERFLOW | ST_EXCEPTION_COINCIDENT | ST_DIVERGED,
 };
 
 // Immutable per-system description (host-written).
@@ -105,6 +107,7 @@ enum : int {
     FAULT_RATTLE_3 = 4,     // RATTLE before the second OU half-step
     FAULT_RATTLE_4 = 5,     // final RATTLE
     FAULT_NONFINITE = 6,    // non-finite position or velocity
+    FAULT_DIVERGED = 7,     // |x| > MAX_COORD in some coordinate after the drift
 };
 
 // A rebuild re-sorts the atoms into blocks when the host asked for it or on

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _PKG = os.path.dirname(_HERE)
 
 OK = 0
-E_INVALID, E_CAPACITY, E_HIP, E_NONFINITE, E_CONSTRAINT, E_NOMEM, E_UNSUPPORTED, E_NODEVICE = range(-1, -9, -1)
+E_INVALID, E_CAPACITY, E_HIP, E_NONFINITE, E_CONSTRAINT, E_NOMEM, E_UNSUPPORTED, E_NODEVICE, E_DIVERGED = range(-1, -10, -1)
 TERM_BOND, TERM_ANGLE, TERM_DIHEDRAL, TERM_NONBONDED, TERM_EXCEPTION, TERM_ALL = 1, 2, 4, 8, 16, 31
 TERM_NAMES = ("bond", "angle", "dihedral", "nonbonded", "exception")
 PAIRS_REFERENCE, PAIRS_BRUTE = 0, 1
@@ -145,6 +145,8 @@ def md_lib():
             "prism_md_compute_forces_fixed": ([H, C.c_uint, C.POINTER(C.c_int64)], C.c_int),
             "prism_md_step": ([H, C.c_uint64, _f32p], C.c_int),
             "prism_md_energies": ([H, C.POINTER(C.c_double)], C.c_int),
+            "prism_md_kinetic_temperature": ([H, C.POINTER(C.c_double)], C.c_int),
+            "prism_md_batch_kinetic_temperature": ([H, C.c_int32, C.POINTER(C.c_double)], C.c_int),
             "prism_md_get_stats": ([H, C.POINTER(StatsV1)], C.c_int),
             "prism_md_last_error": ([H], C.c_char_p),
             "prism_md_count_pairs": ([H, C.POINTER(C.c_uint64)], C.c_int),
@@ -442,9 +444,13 @@ class FusedEngine:
         return F.reshape(-1, 3)
 
     def energies(self):
+        """Potential and kinetic energy and the cumulative work ledger W
+        (kcal/mol), plus the kinetic temperature (K)."""
         out = (C.c_double * 3)()
         self._check(self.lib.prism_md_energies(self.h, out), "energies")
-        return {"potential": out[0], "kinetic": out[1], "temperature": out[2]}
+        T = C.c_double()
+        self._check(self.lib.prism_md_kinetic_temperature(self.h, C.byref(T)), "kinetic_temperature")
+        return {"potential": out[0], "kinetic": out[1], "work": out[2], "temperature": T.value}
 
     # -- stepping
     def step(self, n_steps: int = 1, noise=None):
@@ -590,7 +596,9 @@ class Ensemble:
     def energies(self, s: int):
         out = (C.c_double * 3)()
         self._check(self.lib.prism_md_batch_energies(self.h, s, out), "batch_energies")
-        return {"potential": out[0], "kinetic": out[1], "temperature": out[2]}
+        T = C.c_double()
+        self._check(self.lib.prism_md_batch_kinetic_temperature(self.h, s, C.byref(T)), "batch_kinetic_temperature")
+        return {"potential": out[0], "kinetic": out[1], "work": out[2], "temperature": T.value}
 
     def checkpoint(self, s: int) -> bytes:
         n = C.c_size_t()

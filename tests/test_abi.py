@@ -72,7 +72,7 @@ def test_struct_layouts_match_ctypes(pm):
 
 def test_defaults_and_version(pm):
     lib = pm.md_lib()
-    assert lib.prism_md_abi_version() == 1
+    assert lib.prism_md_abi_version() == 2
     p = pm.default_params()
     assert (p.dt_ps, p.temperature_K, p.cutoff_A, p.skin_A, p.rebuild_every) == (0.004, 300.0, 10.0, 2.0, 20)
     assert (p.gamma_ps, p.equilibration_steps, p.equilibration_gamma_ps) == (10.0, 10000, 1000.0)

@@ -142,3 +142,18 @@ def test_minimize_lowers_energy(pm):
                 continue
             r = np.linalg.norm(x[h[ok, s]] - x[c[ok]], axis=1)
             assert np.abs(r - d[ok, s]).max() < 1e-4
+
+
+def test_pair_on_the_list_edge_is_kept(pm):
+    """Regression: the list test's r2 must round after every operation (no
+    fused multiply-add), so a pair at FP32 r2 = 144.0 exactly stays in the
+    list in both modes, as in the reference and the oracle (the coordinates
+    are those of tests/test_oracle_pairs.py::R2_EDGE)."""
+    import sys
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    from toy import make_topology
+    from test_oracle_pairs import R2_EDGE
+    x = np.array(R2_EDGE, np.float32)
+    with pm.FusedEngine(make_topology(pm, x)) as eng:
+        for mode in (0, 1):
+            assert eng.pairs(mode).tolist() == [[0, 1]], mode

@@ -149,3 +149,40 @@ def test_constrained_nve_second_order(pm, oracle, small_top):
     e2 = _nve_error(pm, oracle, small_top, 0.00025, 0.2)
     assert e2 < 0.4 * e1
     assert e2 < 0.5
+
+
+def test_work_ledger_closes_the_energy_balance(pm, oracle):
+    """W books every non-Hamiltonian kinetic-energy change (OU half-steps,
+    the RATTLE after OU_N and the final RATTLE, force and velocity clamps;
+    nhs_amber_fused.cu:1713-1788, :2245-2270, :11148-11186 with the routing of
+    fused_engine.rs:19273-19277).  The remaining moves -- kicks, drift, the
+    pre-drift and pre-OU_NP1 RATTLE and SHAKE -- are the constrained
+    Hamiltonian flow, so E(t) - E(0) - W is the integrator's shadow error:
+    second order in dt, while W itself is hundreds of kcal/mol.  A dropped,
+    extra or sign-flipped entry leaves a residual of the size of W."""
+    top = pm.Topology.generate(192, seed=11)
+    x0 = top["positions"].reshape(-1, 3).astype(np.float64)
+    v0 = pm.maxwell_boltzmann(top["masses"], 300.0, 5).astype(np.float64)
+    pe0, ke0 = oracle.energies(top, x0, v0)
+    res = []
+    for dt in (0.001, 0.0005):
+        p = pm.default_params(dt_ps=dt, gamma_ps=20.0, equilibration_steps=0, com_remove_every=0)
+        x, v, w = oracle.run(top, p, x0, v0, 0, int(round(0.2 / dt)))
+        pe1, ke1 = oracle.energies(top, x, v)
+        res.append((pe1 + ke1) - (pe0 + ke0) - w)
+        assert abs(w) > 100.0
+        print(f"dt {dt}: W {w:.3f}, dE - W {res[-1]:.4f} kcal/mol")
+    assert abs(res[1]) < 1e-3 * abs(w)
+    assert abs(res[0]) / abs(res[1]) > 3.0  # second order (4 in the limit)
+
+
+def test_work_ledger_is_zero_without_bath_or_clamps(pm, oracle):
+    """gamma = 0: exact kicks, no OU, no RATTLE-after-OU change (the state is
+    already constrained), and no velocity clamp at thermal speeds: W == 0."""
+    top = pm.Topology.generate(192, seed=11)
+    p = pm.default_params(dt_ps=0.001, gamma_ps=0.0, equilibration_steps=0, com_remove_every=0)
+    x = top["positions"].reshape(-1, 3).astype(np.float64)
+    v = pm.maxwell_boltzmann(top["masses"], 300.0, 5).astype(np.float64)
+    x, v, _ = oracle.run(top, p, x, v, 0, 1)  # project the velocities onto the constraints
+    x, v, w = oracle.run(top, p, x, v, 1, 50)
+    assert abs(w) < 1e-9

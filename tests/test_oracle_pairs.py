@@ -137,3 +137,25 @@ def test_capacity_neighbour_row(pm, oracle, rng):
     assert counts.max() <= 512 and len(counts) == 8
     assert oracle.pairs(t, x, x, 0) == -2
     assert oracle.pairs(t, x, x, 1).shape[0] > 0  # brute mode has no row capacity
+
+
+# Two atoms whose FP32 r2, rounded after every operation as the reference's
+# list test rounds it ((dx*dx + dy*dy) + dz*dz, nhs_neighbor_warp.cuh:60-63),
+# is exactly 144.0 -- kept by !(r2 > 144) -- while any fused multiply-add
+# ordering gives 144.00002.  A compiler that contracts the list test drops it.
+R2_EDGE = ([10.0, -20.0, 30.0], [18.26969337463379, -12.415456771850586, 34.25286865234375])
+
+
+def test_pair_on_the_list_edge_is_kept(pm, oracle):
+    import sys
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    from toy import make_topology
+    x = np.array(R2_EDGE, np.float32)
+    d = (x[1] - x[0]).astype(np.float32)
+    sq = (d * d).astype(np.float32)
+    assert np.float32(np.float32(sq[0] + sq[1]) + sq[2]) == np.float32(144.0)
+    fused = np.float32(np.float64(d[2]) * d[2] + np.float64(np.float32(np.float64(d[1]) * d[1] + np.float64(sq[0]))))
+    assert fused > np.float32(144.0)
+    top = make_topology(pm, x)
+    for mode in (0, 1):
+        assert oracle.pairs(top, x, x, mode).tolist() == [[0, 1]]
