@@ -225,9 +225,25 @@ int prism_md_set_state(PrismMdHandle *h, const float *x, const float *v, int fla
 int prism_md_get_state(PrismMdHandle *h, float *x, float *v, int flags);
 
 /* Rebuild the cell hash and tile list at the current positions, then
- * materialise the atom-pair list of the given mode for parity checks.
+ * materialise the atom-pair list of the given mode for parity checks.  This
+ * diagnostic call allocates device staging for the list (hipMalloc/hipFree
+ * of 8 bytes per pair) on each call; prism_md_build_pairs_static does not.
  * Replaces rebuild_neighbor_lists, fused_engine.rs:16615-16748. */
 int prism_md_build_pairs(PrismMdHandle *h, int mode);
+
+/* prism_md_build_pairs without any device allocation, for callers that
+ * cannot allocate (code that runs where hipMalloc is forbidden: tight serving
+ * loops, harnesses that pre-plan all memory).  No counterpart in the
+ * reference.  The pairs are staged in one static device array of the library
+ * (a counter and 4,194,304 pairs, 32 MiB, held by each GPU the library runs
+ * on until the process ends) instead of memory allocated per call; a list of
+ * more pairs is refused with PRISM_MD_E_CAPACITY (n_out of get_pairs is then
+ * 0).  prism_md_get_pairs reads the result as after prism_md_build_pairs.
+ * Every call on one GPU shares that array, which keeps the last call's
+ * contents: calls that overlap on one GPU (two host threads, two handles on
+ * different streams) race.  Calls in sequence are safe; each clears the
+ * counter it uses. */
+int prism_md_build_pairs_static(PrismMdHandle *h, int mode);
 
 /* Copy the pair list of the last prism_md_build_pairs as unique pairs i < j
  * (atom ids of the topology), sorted by (i, j).  *n_out receives the total

@@ -4,13 +4,50 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <utility>
 
 namespace prism {
 
+// Staging of prism_md_build_pairs_static: the library's own device memory for
+// callers that cannot allocate.  Layout: the pair counter (u64) at byte 0, the
+// first atoms of STATIC_PAIRS pairs from byte 256, the second atoms after them.
+// Each GPU the library runs on holds one copy (32 MiB + 256 B) until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: its per-device address comes from hipGetSymbolAddress
+// (static_pair_staging) and only that address is handed to kernels and copies.
+constexpr size_t STATIC_PAIRS = (size_t)1 << 22;  // 4,194,304 pairs
+constexpr size_t STATIC_BYTES = 256 + 2 * STATIC_PAIRS * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[STATIC_BYTES];
+
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+hipError_t static_pair_staging(void **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return hipSuccess;
+}
+
+// Clears the staging's counter: a kernel, not a memset node, so the call has
+// no memset in it.
+__global__ void k_clear_count(unsigned long long *count) { *count = 0ull; }
 
 int pad64(int n) { return (n + BLOCK - 1) / BLOCK * BLOCK; }
 
@@ -186,6 +223,8 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
         if (int rc = hipcheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream")) return rc;
         own_stream_ = true;
     }
+    // the static pair staging of this device, looked up once per device
+    if (int rc = hipcheck(static_pair_staging(&static_staging_), "static pair staging address")) return rc;
     int A = 0, W = 0;
     size_t NB = 0, NA = 0, ND = 0, NE = 0, SK = 0, EX = 0, CS = 0;
     slots_.clear();
@@ -1044,32 +1083,19 @@ int Engine::temperature(int s, double *T) {
     return PRISM_MD_OK;
 }
 
-int Engine::build_pairs(int s, int mode) {
-    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
-    if (mode != PRISM_MD_PAIRS_REFERENCE && mode != PRISM_MD_PAIRS_BRUTE) return fail(PRISM_MD_E_INVALID, "pair mode");
+// The pair list of system s in the given mode, emitted into device staging
+// (i_d, j_d, counter count_d, at most cap pairs) and kept sorted on the host.
+int Engine::emit_pairs(int s, int mode, int *i_d, int *j_d, unsigned long long *count_d, size_t cap) {
     const HostSys &H = sys_[s];
-    if (mode == PRISM_MD_PAIRS_REFERENCE && !H.grid_ok)
-        return fail(PRISM_MD_E_CAPACITY, "reference cell grid needs more than 32 cells per axis");
-    if (int rc = refresh_all()) return rc;
-    if (int rc = check_status()) return rc;
+    hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(1), 0, stream_, count_d);
+    launch_emit_pairs(P_, mode, H.grid, i_d, j_d, count_d, (unsigned long long)cap, stream_);
     unsigned long long count = 0;
-    hipMemsetAsync(d_count_, 0, sizeof(count), stream_);
-    launch_emit_pairs(P_, mode, H.grid, nullptr, nullptr, d_count_, 0ull, stream_);
-    if (int rc = d2h(&count, d_count_, sizeof(count))) return rc;
-    int *di = nullptr, *dj = nullptr;
-    if (int rc = hipcheck(hipMalloc(&di, std::max<size_t>(1, count) * sizeof(int)), "pair alloc")) return rc;
-    if (int rc = hipcheck(hipMalloc(&dj, std::max<size_t>(1, count) * sizeof(int)), "pair alloc")) {
-        hipFree(di);
-        return rc;
-    }
-    hipMemsetAsync(d_count_, 0, sizeof(count), stream_);
-    launch_emit_pairs(P_, mode, H.grid, di, dj, d_count_, count, stream_);
+    if (int rc = d2h(&count, count_d, sizeof(count))) return rc;
+    if (count > cap) return fail(PRISM_MD_E_CAPACITY, "pair list of " + std::to_string(count) +
+                                                          " pairs exceeds the staging of " + std::to_string(cap));
     std::vector<int> hi(count), hj(count);
-    int rc = d2h(hi.data(), di, count * sizeof(int));
-    if (!rc) rc = d2h(hj.data(), dj, count * sizeof(int));
-    hipFree(di);
-    hipFree(dj);
-    if (rc) return rc;
+    if (int rc = d2h(hi.data(), i_d, count * sizeof(int))) return rc;
+    if (int rc = d2h(hj.data(), j_d, count * sizeof(int))) return rc;
     // the reference audits per-atom list saturation (fused_engine.rs:16722-16735)
     std::vector<std::pair<int, int>> pr(count);
     std::vector<int> degree(H.n_atoms, 0);
@@ -1089,6 +1115,47 @@ int Engine::build_pairs(int s, int mode) {
         pairs_j_[k] = pr[k].second;
     }
     return PRISM_MD_OK;
+}
+
+// Rebuild the lists at the current positions (parity entry points).
+int Engine::pairs_prologue(int s, int mode) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    if (mode != PRISM_MD_PAIRS_REFERENCE && mode != PRISM_MD_PAIRS_BRUTE) return fail(PRISM_MD_E_INVALID, "pair mode");
+    if (mode == PRISM_MD_PAIRS_REFERENCE && !sys_[s].grid_ok)
+        return fail(PRISM_MD_E_CAPACITY, "reference cell grid needs more than 32 cells per axis");
+    pairs_i_.clear();
+    pairs_j_.clear();
+    if (int rc = refresh_all()) return rc;
+    return check_status();
+}
+
+int Engine::build_pairs(int s, int mode) {
+    if (int rc = pairs_prologue(s, mode)) return rc;
+    // count first, then stage exactly that many pairs in memory of this call
+    hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(1), 0, stream_, d_count_);
+    launch_emit_pairs(P_, mode, sys_[s].grid, nullptr, nullptr, d_count_, 0ull, stream_);
+    unsigned long long count = 0;
+    if (int rc = d2h(&count, d_count_, sizeof(count))) return rc;
+    int *di = nullptr, *dj = nullptr;
+    if (int rc = hipcheck(hipMalloc(&di, std::max<size_t>(1, count) * sizeof(int)), "pair alloc")) return rc;
+    if (int rc = hipcheck(hipMalloc(&dj, std::max<size_t>(1, count) * sizeof(int)), "pair alloc")) {
+        hipFree(di);
+        return rc;
+    }
+    const int rc = emit_pairs(s, mode, di, dj, d_count_, count);
+    hipFree(di);
+    hipFree(dj);
+    return rc;
+}
+
+// The same list staged in the library's static device array: no allocation.
+int Engine::build_pairs_static(int s, int mode) {
+    if (!static_staging_) return fail(PRISM_MD_E_HIP, "static pair staging not set up");
+    if (int rc = pairs_prologue(s, mode)) return rc;
+    char *base = static_cast<char *>(static_staging_);
+    int *si = reinterpret_cast<int *>(base + 256);
+    int *sj = si + STATIC_PAIRS;
+    return emit_pairs(s, mode, si, sj, reinterpret_cast<unsigned long long *>(base), STATIC_PAIRS);
 }
 
 int Engine::get_pairs(int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out) {
@@ -1472,6 +1539,9 @@ int prism_md_get_state(PrismMdHandle *h, float *x, float *v, int flags) {
     return h ? h->eng.get_state(0, x, v, flags) : PRISM_MD_E_INVALID;
 }
 int prism_md_build_pairs(PrismMdHandle *h, int mode) { return h ? h->eng.build_pairs(0, mode) : PRISM_MD_E_INVALID; }
+int prism_md_build_pairs_static(PrismMdHandle *h, int mode) {
+    return h ? h->eng.build_pairs_static(0, mode) : PRISM_MD_E_INVALID;
+}
 int prism_md_get_pairs(PrismMdHandle *h, int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out) {
     return h ? h->eng.get_pairs(i_out, j_out, cap, n_out) : PRISM_MD_E_INVALID;
 }

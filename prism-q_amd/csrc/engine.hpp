@@ -50,6 +50,8 @@ class Engine {
     int energies(int s, double out[3]);   // potential, kinetic, cumulative work
     int temperature(int s, double *T);    // kinetic temperature over the system's degrees of freedom
     int build_pairs(int s, int mode);
+    // build_pairs staged in the library's static device array (no allocation)
+    int build_pairs_static(int s, int mode);
     int get_pairs(int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out);
     int count_pairs(int s, uint64_t *n);
     int time_nonbonded(int iters, double *mean_ms);
@@ -80,6 +82,8 @@ class Engine {
     int hipcheck(hipError_t e, const char *what);
     int d2h(void *dst, const void *src, size_t bytes);  // stream-ordered copy + synchronise
     int reduce_system(int s, double red[RED_STRIDE]);
+    int pairs_prologue(int s, int mode);
+    int emit_pairs(int s, int mode, int *i_d, int *j_d, unsigned long long *count_d, size_t cap);
     int refresh_all();            // rebuild lists + forces at current positions
     int refresh(int only);        // the same for one system (-1: all)
     int layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream);
@@ -159,6 +163,7 @@ class Engine {
     float4 *d_vmin_ = nullptr;    // FIRE velocities
     double *d_fire_ = nullptr;    // FIRE partials
     SysDesc *d_desc_ = nullptr;
+    void *static_staging_ = nullptr;  // this device's address of the static pair staging
     hipGraphExec_t graph_ = nullptr;
     int graph_steps_ = 0;
     uint64_t host_steps_ = 0;

@@ -140,6 +140,7 @@ def md_lib():
             "prism_md_set_state": ([H, _f32p, _f32p, C.c_int], C.c_int),
             "prism_md_get_state": ([H, _f32p, _f32p, C.c_int], C.c_int),
             "prism_md_build_pairs": ([H, C.c_int], C.c_int),
+            "prism_md_build_pairs_static": ([H, C.c_int], C.c_int),
             "prism_md_get_pairs": ([H, _i32p, _i32p, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
             "prism_md_compute_forces": ([H, C.c_uint, _f32p, C.POINTER(C.c_double)], C.c_int),
             "prism_md_compute_forces_fixed": ([H, C.c_uint, C.POINTER(C.c_int64)], C.c_int),
@@ -419,8 +420,14 @@ class FusedEngine:
     def rebuild_neighbor_lists(self, mode: int = PAIRS_BRUTE):
         self._check(self.lib.prism_md_build_pairs(self.h, mode), "build_pairs")
 
-    def pairs(self, mode: int = PAIRS_BRUTE) -> np.ndarray:
-        self.rebuild_neighbor_lists(mode)
+    def pairs(self, mode: int = PAIRS_BRUTE, static: bool = False) -> np.ndarray:
+        """Unique pairs (i < j) of the list built at the current positions;
+        static=True stages them in the library's static device array
+        (prism_md_build_pairs_static, no allocation)."""
+        if static:
+            self._check(self.lib.prism_md_build_pairs_static(self.h, mode), "build_pairs_static")
+        else:
+            self.rebuild_neighbor_lists(mode)
         n = C.c_size_t()
         self.lib.prism_md_get_pairs(self.h, None, None, 0, C.byref(n))
         i = np.zeros(max(1, n.value), np.int32)
