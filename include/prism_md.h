@@ -377,6 +377,31 @@ int prism_md_batch_retire(PrismMdBatch *b, int32_t slot);
 int prism_md_batch_slot_info(PrismMdBatch *b, int32_t slot, PrismMdSlotCapsV1 *caps, int32_t *occupied);
 int prism_md_batch_device_bytes(PrismMdBatch *b, uint64_t *bytes);
 
+/* ---- step limits and per-step latency ----------------------------------------
+ * A system stops advancing when its step count reaches end_step (0 removes
+ * the limit): later steps and graph replays leave it exactly as it is, so a
+ * job of N steps ends at step N even when it shares replays of many steps
+ * with other jobs.  Counterpart of the reference's per-run step budget
+ * (NhsAmberFusedEngine::run(n), fused_engine.rs:23911).
+ *
+ * Per-step latency comes from the device: the integrator's last workgroup
+ * stamps the end of every step with the device's constant-rate wall clock,
+ * and a system's step latency is the time from the end of the previous step
+ * to the end of its own, on the device clock -- including whatever ran
+ * between them (other launches, admissions and retirements of other jobs,
+ * host gaps).  Latencies are kept per system from its admission (its first
+ * step, with no previous step of its own, is not counted); the most recent
+ * 4,096 step ends are buffered on the device, so a single step call of more
+ * steps keeps only its last 4,096.  No reference counterpart: the reference's
+ * step timings are profiler receipts outside its tree (README.md:105-107). */
+int prism_md_set_step_limit(PrismMdHandle *h, uint64_t end_step);
+int prism_md_query_latency(PrismMdHandle *h, double *p50_ms, double *p99_ms, uint64_t *samples);
+int prism_md_batch_set_step_limit(PrismMdBatch *b, int32_t system, uint64_t end_step);
+int prism_md_batch_query_latency(PrismMdBatch *b, int32_t system, double *p50_ms, double *p99_ms,
+                                 uint64_t *samples);
+/* The latencies themselves (ms, in step order, at most cap); *n = how many exist. */
+int prism_md_batch_latency_samples(PrismMdBatch *b, int32_t system, float *ms, size_t cap, size_t *n);
+
 /* ---- checkpoint / restore ---------------------------------------------------
  * The dynamic state of a system -- positions, velocities, the positions and
  * block boxes of the last pair-list build, the sorted atom order, the step

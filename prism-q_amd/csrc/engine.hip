@@ -225,6 +225,13 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     }
     // the static pair staging of this device, looked up once per device
     if (int rc = hipcheck(static_pair_staging(&static_staging_), "static pair staging address")) return rc;
+    {
+        int dev = 0, khz = 0;
+        if (int rc = hipcheck(hipGetDevice(&dev), "device")) return rc;
+        if (int rc = hipcheck(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate"))
+            return rc;
+        ms_per_tick_ = khz > 0 ? 1.0 / (double)khz : 0.0;
+    }
     int A = 0, W = 0;
     size_t NB = 0, NA = 0, ND = 0, NE = 0, SK = 0, EX = 0, CS = 0;
     slots_.clear();
@@ -288,6 +295,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     h.inc_idx.assign(std::max<size_t>(CS, 1), 0);
     sys_.assign(n_sys, HostSys{});
     desc_.assign(n_sys, SysDesc{});
+    lat_.assign(n_sys, LatencyLog{});
     for (int k = 0; k < n_sys; ++k) {
         desc_[k].atom_begin = slots_[k].atom_base;
         desc_[k].block_begin = slots_[k].atom_base / BLOCK;
@@ -357,6 +365,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     P_.work_g = dalloc<double>((size_t)W * 64);
     P_.sweeps = dalloc<int>((size_t)W * 2);
     P_.counters = dalloc<EngineCounters>(1);
+    P_.step_ts = dalloc<unsigned long long>(TS_RING);
     P_.hs_group = dalloc<unsigned>((size_t)W / 64 + 2);  // >= second-half workgroups / 128, zeroed
     d_count_ = dalloc<unsigned long long>(1);
     d_red_ = dalloc<double>((size_t)n_sys * RED_STRIDE);
@@ -667,6 +676,7 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     D.exc_begin = L.exc_base;
     D.n_exc = t->n_exceptions;
     desc_[s] = D;
+    lat_[s] = LatencyLog{};
     H.topo_digest = topology_digest(t);
     H.params_digest = params_digest(D);
     sys_[s] = std::move(H);
@@ -842,10 +852,85 @@ void Engine::launch_step(const float *noise_n, const float *noise_np1) {
     ++force_evals_;
 }
 
-int Engine::check_status() {
+// Turn the device's step-end stamps since the last harvest into per-step
+// latencies of every system that advanced.  Systems only stop (limit, fault)
+// or start (admission, activation) between host calls, so a system that
+// advanced d steps in a window of global steps [lo, hi) ran its first d.  A
+// step's latency is the time from the previous step's end to its own: it
+// includes whatever the device or the host did between the two (admissions,
+// retirements, copies).  A system's first step after admission has no
+// previous step of its own and is not counted.  Only the TS_RING most recent
+// stamps are kept, so a single call of more steps loses its older ones.
+int Engine::harvest_step_ends(const std::vector<SysState> &st) {
+    unsigned long long hi = 0;
+    if (int rc = d2h(&hi, &P_.counters->step_ends, sizeof(hi))) return rc;
+    const unsigned long long lo = ts_harvested_;
+    if (hi <= lo) return PRISM_MD_OK;
+    // stamps [first, hi) are still in the ring; one before lo gives lo's latency
+    const unsigned long long first = std::max<unsigned long long>(lo == 0 ? 0 : lo - 1, hi > TS_RING ? hi - TS_RING : 0);
+    std::vector<unsigned long long> ts(hi - first);
+    for (unsigned long long k = first; k < hi;) {  // at most two contiguous pieces of the ring
+        const unsigned long long idx = k % TS_RING, len = std::min<unsigned long long>(hi - k, TS_RING - idx);
+        if (int rc = d2h(ts.data() + (k - first), P_.step_ts + idx, len * sizeof(unsigned long long))) return rc;
+        k += len;
+    }
+    for (size_t s = 0; s < sys_.size(); ++s) {
+        LatencyLog &L = lat_[s];
+        const unsigned long long now = st[s].step;
+        if (!desc_[s].active || now <= L.steps_seen) {
+            L.steps_seen = now;
+            continue;
+        }
+        const unsigned long long d = now - L.steps_seen;
+        for (unsigned long long k = lo; k < lo + d && k < hi; ++k) {
+            const bool first_of_system = L.steps_seen == 0 && k == lo;
+            if (first_of_system || k == 0 || k - 1 < first) continue;
+            L.ms.push_back((float)((double)(ts[k - first] - ts[k - 1 - first]) * ms_per_tick_));
+        }
+        L.steps_seen = now;
+    }
+    ts_harvested_ = hi;
+    return PRISM_MD_OK;
+}
+
+int Engine::latency(int s, double *p50, double *p99, uint64_t *n, float *samples, size_t cap) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    std::vector<SysState> st;  // harvest what the device has done; a latched fault is no error here
+    if (int rc = sync_states(st)) return rc;
+    const std::vector<float> &v = lat_[s].ms;
+    if (n) *n = v.size();
+    if (samples) std::copy(v.begin(), v.begin() + std::min(cap, v.size()), samples);
+    auto pct = [&](double q) {
+        if (v.empty()) return 0.0;
+        std::vector<float> w(v);
+        const size_t k = std::min(w.size() - 1, (size_t)std::floor(q * (double)(w.size() - 1) + 0.5));
+        std::nth_element(w.begin(), w.begin() + (ptrdiff_t)k, w.end());
+        return (double)w[k];
+    };
+    if (p50) *p50 = pct(0.50);
+    if (p99) *p99 = pct(0.99);
+    return PRISM_MD_OK;
+}
+
+int Engine::set_step_limit(int s, uint64_t end_step) {
+    if (s < 0 || s >= (int)sys_.size()) return fail(PRISM_MD_E_INVALID, "system index");
+    desc_[s].end_step = end_step;
+    return hipcheck(hipMemcpyAsync((char *)(d_desc_ + s) + offsetof(SysDesc, end_step), &desc_[s].end_step,
+                                   sizeof(desc_[s].end_step), hipMemcpyHostToDevice, stream_),
+                    "step limit");
+}
+
+// Wait for the stream, read every system's state and harvest the step ends.
+int Engine::sync_states(std::vector<SysState> &st) {
     if (int rc = hipcheck(hipStreamSynchronize(stream_), "synchronize")) return rc;
-    std::vector<SysState> st(sys_.size());
+    st.resize(sys_.size());
     if (int rc = d2h(st.data(), P_.sst, st.size() * sizeof(SysState))) return rc;
+    return harvest_step_ends(st);
+}
+
+int Engine::check_status() {
+    std::vector<SysState> st;
+    if (int rc = sync_states(st)) return rc;
     for (size_t s = 0; s < st.size(); ++s) {
         const int b = st[s].status;
         // a retired or deactivated system keeps its latched fault for
@@ -1317,7 +1402,9 @@ int Engine::set_active(int s, bool active) {
         // drop its units: every active block re-claims, no row changes
         EngineCounters c{};
         c.any_dirty = 1;
-        if (int rc = hipcheck(hipMemcpyAsync(P_.counters, &c, sizeof(c), hipMemcpyHostToDevice, stream_), "counters"))
+        // the flags only: the step-end count behind them keeps running
+        if (int rc = hipcheck(hipMemcpyAsync(P_.counters, &c, offsetof(EngineCounters, pad), hipMemcpyHostToDevice, stream_),
+                              "counters"))
             return rc;
         launch_rows(P_, stream_);
         launch_consume(P_, -1, stream_);
@@ -1453,7 +1540,8 @@ int Engine::checkpoint_restore(int s, const void *buf, size_t len) {
     EngineCounters c{};
     c.any_dirty = 1;
     if ((rc = hipcheck(hipMemcpyAsync(P_.sst + s, &st, sizeof st, hipMemcpyHostToDevice, stream_), "restore state")) ||
-        (rc = hipcheck(hipMemcpyAsync(P_.counters, &c, sizeof c, hipMemcpyHostToDevice, stream_), "restore counters")))
+        (rc = hipcheck(hipMemcpyAsync(P_.counters, &c, offsetof(EngineCounters, pad), hipMemcpyHostToDevice, stream_),
+                       "restore counters")))
         return rc;
     launch_rows(P_, stream_);
     SysState after;
@@ -1539,6 +1627,14 @@ int prism_md_get_state(PrismMdHandle *h, float *x, float *v, int flags) {
     return h ? h->eng.get_state(0, x, v, flags) : PRISM_MD_E_INVALID;
 }
 int prism_md_build_pairs(PrismMdHandle *h, int mode) { return h ? h->eng.build_pairs(0, mode) : PRISM_MD_E_INVALID; }
+int prism_md_set_step_limit(PrismMdHandle *h, uint64_t end_step) {
+    if (!h) return PRISM_MD_E_INVALID;
+    if (int rc = h->eng.set_step_limit(0, end_step)) return rc;
+    return h->eng.check_status();
+}
+int prism_md_query_latency(PrismMdHandle *h, double *p50_ms, double *p99_ms, uint64_t *samples) {
+    return h ? h->eng.latency(0, p50_ms, p99_ms, samples, nullptr, 0) : PRISM_MD_E_INVALID;
+}
 int prism_md_build_pairs_static(PrismMdHandle *h, int mode) {
     return h ? h->eng.build_pairs_static(0, mode) : PRISM_MD_E_INVALID;
 }
@@ -1638,6 +1734,21 @@ int prism_md_batch_compute_forces(PrismMdBatch *b, int32_t system, unsigned term
 
 int prism_md_batch_kinetic_temperature(PrismMdBatch *b, int32_t system, double *T) {
     return b && T ? b->eng.temperature(system, T) : PRISM_MD_E_INVALID;
+}
+int prism_md_batch_set_step_limit(PrismMdBatch *b, int32_t system, uint64_t end_step) {
+    if (!b) return PRISM_MD_E_INVALID;
+    if (int rc = b->eng.set_step_limit(system, end_step)) return rc;
+    return b->eng.check_status();
+}
+int prism_md_batch_query_latency(PrismMdBatch *b, int32_t system, double *p50_ms, double *p99_ms, uint64_t *samples) {
+    return b ? b->eng.latency(system, p50_ms, p99_ms, samples, nullptr, 0) : PRISM_MD_E_INVALID;
+}
+int prism_md_batch_latency_samples(PrismMdBatch *b, int32_t system, float *ms, size_t cap, size_t *n) {
+    if (!b || !n || (cap && !ms)) return PRISM_MD_E_INVALID;
+    uint64_t k = 0;
+    if (int rc = b->eng.latency(system, nullptr, nullptr, &k, ms, cap)) return rc;
+    *n = (size_t)k;
+    return PRISM_MD_OK;
 }
 int prism_md_batch_energies(PrismMdBatch *b, int32_t system, double out[3]) {
     return b && out ? b->eng.energies(system, out) : PRISM_MD_E_INVALID;

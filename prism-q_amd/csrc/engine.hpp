@@ -50,6 +50,11 @@ class Engine {
     int energies(int s, double out[3]);   // potential, kinetic, cumulative work
     int temperature(int s, double *T);    // kinetic temperature over the system's degrees of freedom
     int build_pairs(int s, int mode);
+    // per-step latency of system s from the device's step-end stamps (ms);
+    // samples (cap entries at most) may be NULL
+    int latency(int s, double *p50, double *p99, uint64_t *n, float *samples, size_t cap);
+    // system s stops advancing when its step count reaches end_step (0: no limit)
+    int set_step_limit(int s, uint64_t end_step);
     // build_pairs staged in the library's static device array (no allocation)
     int build_pairs_static(int s, int mode);
     int get_pairs(int32_t *i_out, int32_t *j_out, size_t cap, size_t *n_out);
@@ -83,6 +88,15 @@ class Engine {
     int d2h(void *dst, const void *src, size_t bytes);  // stream-ordered copy + synchronise
     int reduce_system(int s, double red[RED_STRIDE]);
     int pairs_prologue(int s, int mode);
+    int harvest_step_ends(const std::vector<SysState> &st);
+    int sync_states(std::vector<SysState> &st);
+    struct LatencyLog {
+        unsigned long long steps_seen = 0;  // the system's step count at the last harvest
+        std::vector<float> ms;              // per-step latencies since admission
+    };
+    std::vector<LatencyLog> lat_;
+    unsigned long long ts_harvested_ = 0;   // step ends already turned into latencies
+    double ms_per_tick_ = 0.0;              // device wall clock period
     int emit_pairs(int s, int mode, int *i_d, int *j_d, unsigned long long *count_d, size_t cap);
     int refresh_all();            // rebuild lists + forces at current positions
     int refresh(int only);        // the same for one system (-1: all)

@@ -84,6 +84,7 @@ struct DevPtrs {
     int *sweeps;                    // [waves][2]: most SHAKE / RATTLE sweeps so far, per group wave
                                     // (written only by that wave; reduced on the host)
     EngineCounters *counters;
+    unsigned long long *step_ts;    // [TS_RING] device wall clock at the end of step k (slot k % TS_RING)
 };
 
 struct RefGrid {                    // reference cell grid of the pair-list parity mode

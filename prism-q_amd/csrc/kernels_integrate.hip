@@ -374,7 +374,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
     const SysDesc &D = P.sys[s];
     if (!D.active) return;
     SysState &S = P.sst[s];
-    if (S.halted) {  // frozen, but its forces are still re-evaluated into zeroed accumulators
+    if (S.halted || parked(D, S)) {  // frozen, but its forces are still re-evaluated into zeroed accumulators
         const Group G = P.groups[gw * 64 + lane];
         for (int k = 0; k < 4; ++k)
             if (G.atom[k] >= 0)
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
     __shared__ int last;
     bool publish = false;  // this thread wrote something the last workgroup reads
     const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
-    if (s >= 0 && P.sys[s].active && !P.sst[s].halted) {
+    if (s >= 0 && P.sys[s].active && !P.sst[s].halted && !parked(P.sys[s], P.sst[s])) {
         const SysDesc &D = P.sys[s];
         SysState &S = P.sst[s];
         const unsigned long long step = S.step;
@@ -535,7 +535,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         const SysDesc &D = P.sys[q];
         if (!D.active) continue;
         SysState &S = P.sst[q];
-        if (S.halted) continue;
+        if (S.halted || parked(D, S)) continue;
         const unsigned long long step = S.step;
         if (D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0) {
             double acc[3] = {0.0, 0.0, 0.0};
@@ -557,6 +557,12 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         P.counters->done_groups = 0u;
         P.counters->any_dirty = 0;
         P.counters->any_rebuild = 0;
+        // the step's end on the device's constant-rate clock: the host turns
+        // consecutive stamps into per-step latencies that include every gap
+        // between steps (host work, admissions, other launches)
+        const unsigned long long k = P.counters->step_ends;
+        P.step_ts[k % TS_RING] = (unsigned long long)wall_clock64();
+        P.counters->step_ends = k + 1;
     }
 }
 

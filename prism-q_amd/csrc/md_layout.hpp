@@ -77,6 +77,7 @@ struct SysDesc {
     // term ranges of this system in the global term arrays (energy sums)
     int bond_begin, n_bonds, angle_begin, n_angles;
     int dih_begin, n_dih, exc_begin, n_exc;
+    unsigned long long end_step;    // step count at which the system stops advancing (0: no limit)
 };
 
 // Mutable per-system state (device-written).
@@ -117,6 +118,12 @@ __host__ __device__ inline bool resort_due(const SysDesc &D, const SysState &S) 
     return S.resort_flag || D.resort_every <= 1 || S.n_rebuilds % D.resort_every == 0;
 }
 
+// A system that has reached its step limit keeps its state and does not
+// advance; the integrator treats it like a halted one.
+__host__ __device__ inline bool parked(const SysDesc &D, const SysState &S) {
+    return D.end_step != 0 && S.step >= D.end_step;
+}
+
 // Global engine counters (device-written).
 struct EngineCounters {
     unsigned int done_groups;       // step-end handshake of the integrator: groups of workgroups
@@ -125,8 +132,10 @@ struct EngineCounters {
     int n_units;                    // nonbonded work units claimed by the row builder
     int any_rebuild;                // some system raised its rebuild flag (cleared with any_dirty);
                                     // the rebuild kernels return at once while it is 0
-    int pad[4];
+    int pad[2];
+    unsigned long long step_ends;   // steps completed by the engine (all systems together)
 };
+constexpr int TS_RING = 4096;       // device wall-clock stamps of the most recent step ends
 
 struct Group {                      // constraint group
     int atom[4];                    // central, h0, h1, h2 (-1 unused); atom[0] < 0 => padding

@@ -141,6 +141,13 @@ def md_lib():
             "prism_md_get_state": ([H, _f32p, _f32p, C.c_int], C.c_int),
             "prism_md_build_pairs": ([H, C.c_int], C.c_int),
             "prism_md_build_pairs_static": ([H, C.c_int], C.c_int),
+            "prism_md_set_step_limit": ([H, C.c_uint64], C.c_int),
+            "prism_md_query_latency": ([H, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+                                       C.c_int),
+            "prism_md_batch_set_step_limit": ([H, C.c_int32, C.c_uint64], C.c_int),
+            "prism_md_batch_query_latency": ([H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                              C.POINTER(C.c_uint64)], C.c_int),
+            "prism_md_batch_latency_samples": ([H, C.c_int32, _f32p, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
             "prism_md_get_pairs": ([H, _i32p, _i32p, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
             "prism_md_compute_forces": ([H, C.c_uint, _f32p, C.POINTER(C.c_double)], C.c_int),
             "prism_md_compute_forces_fixed": ([H, C.c_uint, C.POINTER(C.c_int64)], C.c_int),
@@ -491,6 +498,16 @@ class FusedEngine:
         self._check(self.lib.prism_md_kernel_timing(self.h, every, C.byref(ms), C.byref(n)), "kernel_timing")
         return float(ms.value), int(n.value)
 
+    def set_step_limit(self, end_step: int):
+        """Stop advancing at step count end_step (0: no limit)."""
+        self._check(self.lib.prism_md_set_step_limit(self.h, end_step), "set_step_limit")
+
+    def latency(self) -> dict:
+        """p50 / p99 per-step latency (ms) from the device's step-end stamps."""
+        p50, p99, n = C.c_double(), C.c_double(), C.c_uint64()
+        self._check(self.lib.prism_md_query_latency(self.h, C.byref(p50), C.byref(p99), C.byref(n)), "query_latency")
+        return {"p50_ms": p50.value, "p99_ms": p99.value, "samples": int(n.value)}
+
     def checkpoint(self) -> bytes:
         """The system's dynamic state (prism_md_checkpoint_save): restoring it
         into an engine of the same topology and parameters continues the
@@ -606,6 +623,23 @@ class Ensemble:
         T = C.c_double()
         self._check(self.lib.prism_md_batch_kinetic_temperature(self.h, s, C.byref(T)), "batch_kinetic_temperature")
         return {"potential": out[0], "kinetic": out[1], "work": out[2], "temperature": T.value}
+
+    def set_step_limit(self, s: int, end_step: int):
+        self._check(self.lib.prism_md_batch_set_step_limit(self.h, s, end_step), "batch_set_step_limit")
+
+    def latency(self, s: int) -> dict:
+        p50, p99, n = C.c_double(), C.c_double(), C.c_uint64()
+        self._check(self.lib.prism_md_batch_query_latency(self.h, s, C.byref(p50), C.byref(p99), C.byref(n)),
+                    "batch_query_latency")
+        return {"p50_ms": p50.value, "p99_ms": p99.value, "samples": int(n.value)}
+
+    def latency_samples(self, s: int) -> np.ndarray:
+        n = C.c_size_t()
+        self._check(self.lib.prism_md_batch_latency_samples(self.h, s, None, 0, C.byref(n)), "latency_samples")
+        out = np.zeros(max(1, n.value), np.float32)
+        self._check(self.lib.prism_md_batch_latency_samples(self.h, s, _ptr(out, C.c_float), n.value, C.byref(n)),
+                    "latency_samples")
+        return out[: n.value]
 
     def checkpoint(self, s: int) -> bytes:
         n = C.c_size_t()

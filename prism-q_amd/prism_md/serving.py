@@ -1,11 +1,17 @@
 """Continuous-batching serving loop over a ServingPool.
 
 Requests (jobs) arrive while others run.  Each is admitted into the smallest
-free slot whose capacities fit it, as soon as one is free; every tick
-advances all running jobs by one replay of the captured step graph
-(graph_steps steps); a job that has completed its length is retired and its
-slot reused.  Coordinates, velocities, lists and noise counters stay on the
-device from admission to retirement.
+free slot whose capacities fit it, as soon as one is free, with a device step
+limit at its length; every tick advances all running jobs by one replay of
+the captured step graph (graph_steps steps) -- a job that reaches its length
+inside a replay stops there -- and a job that has completed its length is
+retired and its slot reused.  Coordinates, velocities, lists and noise
+counters stay on the device from admission to retirement.
+
+Per-step latency is the device's: the time between consecutive step ends on
+the device clock (prism_md_batch_latency_samples), kept per job from its
+admission, so the gaps that admissions and retirements of other jobs put
+between two of a job's steps are in it.
 
 The pool is sized up front from an HBM budget (plan_slots): slot classes by
 atom capacity, counts following the expected job-size mix, until the budget
@@ -38,6 +44,7 @@ class Job:
     final_velocities: np.ndarray | None = None
     status: int = 0
     error: str | None = None
+    latency_ms: np.ndarray | None = None   # device per-step latencies of the job
 
     @property
     def n_atoms(self) -> int:
@@ -78,7 +85,7 @@ class Scheduler:
         self.running: dict[int, Job] = {}
         self.done: list[Job] = []
         self.failed: list[Job] = []
-        self.step_latency: list[float] = []   # seconds per step, one entry per tick
+        self.tick_latency: list[float] = []   # host seconds per step, one entry per tick
         self.atom_steps = 0
         self.busy = 0.0
         self.n_slots = len(pool.caps)
@@ -103,6 +110,7 @@ class Scheduler:
                 continue
             free.remove(slot)
             self.pool.admit(slot, job.topology, job.params)
+            self.pool.set_step_limit(slot, job.steps)  # stops at its length, even inside a replay
             job.slot, job.t_admit = slot, self.clock()
             self.running[slot] = job
         self.queue = waiting
@@ -121,16 +129,18 @@ class Scheduler:
             fault = ex
         dt = self.clock() - t0
         self.busy += dt
-        self.step_latency.append(dt / self.graph_steps)
+        self.tick_latency.append(dt / self.graph_steps)
         if fault is not None:
             self._retire_faulted(fault)
-        self.atom_steps += self.graph_steps * sum(j.n_atoms for j in self.running.values())
         for slot, job in list(self.running.items()):
-            job.done_steps += self.graph_steps
+            ran = min(self.graph_steps, job.steps - job.done_steps)  # the device limit stops it there
+            job.done_steps += ran
+            self.atom_steps += ran * job.n_atoms
             if job.done_steps >= job.steps:
                 job.final_positions = self.pool.get_positions(slot)
                 job.final_velocities = self.pool.get_velocities(slot)
                 job.status = self.pool.stats(slot)["status"]
+                job.latency_ms = self.pool.latency_samples(slot)
                 self.pool.retire(slot)
                 job.t_done = self.clock()
                 self.done.append(job)
@@ -156,18 +166,32 @@ class Scheduler:
             job.error = str(fault) if len(faulted) == 1 else f"device status {job.status}"
             job.final_positions = self.pool.get_positions(slot)
             job.final_velocities = self.pool.get_velocities(slot)
+            job.latency_ms = self.pool.latency_samples(slot)
             self.pool.retire(slot)
             job.t_done = self.clock()
             self.failed.append(job)
             del self.running[slot]
 
+    def job_latencies(self) -> np.ndarray:
+        """Device per-step latencies of every job, finished or running (one
+        entry per job-step: a step shared by k jobs counts k times)."""
+        parts = [j.latency_ms for j in self.done + self.failed if j.latency_ms is not None]
+        parts += [self.pool.latency_samples(slot) for slot in self.running]
+        return np.concatenate(parts) if parts else np.zeros(0, np.float32)
+
     def report(self, wall: float) -> dict:
-        lat = np.asarray(self.step_latency) * 1e3
+        tick = np.asarray(self.tick_latency) * 1e3
+        lat = self.job_latencies()
         waits = [j.t_admit - j.t_arrive for j in self.done + list(self.running.values()) if j.t_admit is not None]
+        per_job_p99 = [float(np.percentile(j.latency_ms, 99)) for j in self.done if j.latency_ms is not None
+                       and len(j.latency_ms)]
+        pct = lambda a, q: round(float(np.percentile(a, q)), 4) if len(a) else None
         return {
-            "ticks": len(lat), "graph_steps": self.graph_steps,
-            "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
-                                "p99": round(float(np.percentile(lat, 99)), 4) if len(lat) else None},
+            "ticks": len(tick), "graph_steps": self.graph_steps,
+            "step_latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "samples": int(len(lat)),
+                                "source": "device step-end stamps, per job-step, gaps between steps included"},
+            "step_latency_ms_median_job_p99": round(float(np.median(per_job_p99)), 4) if per_job_p99 else None,
+            "tick_latency_ms_host": {"p50": pct(tick, 50), "p99": pct(tick, 99)},
             "atom_steps_per_s": round(self.atom_steps / wall, 1) if wall > 0 else 0.0,
             "gpu_busy_fraction": round(self.busy / wall, 4) if wall > 0 else 0.0,
             "jobs_done": len(self.done), "jobs_running": len(self.running), "jobs_queued": len(self.queue),

@@ -73,16 +73,71 @@ def test_capacity_and_occupancy_errors(pm, jobs, pool):
 def test_scheduler_end_to_end(pm, jobs, pool):
     from prism_md.serving import Job, Scheduler
     sch = Scheduler(pool, graph_steps=10)
-    lengths = [60, 100, 40, 80]
+    lengths = [57, 100, 43, 80]  # two end inside a replay
     for k, steps in enumerate(lengths):
         top, p = jobs[k % 3]
         sch.submit(Job(k, top, p, steps))
     while sch.tick():
         pass
-    assert len(sch.done) == 4 and not sch.report(1.0)["faulted_jobs"]
-    for job in sch.done:
-        x1, v1, _ = alone(pm, job.topology, job.params, job.done_steps)
+    rep = sch.report(1.0)
+    assert len(sch.done) == 4 and not rep["faulted_jobs"]
+    assert rep["step_latency_ms"]["samples"] == sum(n - 1 for n in lengths)
+    for job, n in zip(sorted(sch.done, key=lambda j: j.jid), lengths):
+        assert job.done_steps == n
+        x1, v1, _ = alone(pm, job.topology, job.params, n)
         assert np.array_equal(job.final_positions, x1), job.jid
+
+
+def test_step_limit_stops_inside_a_replay(pm, jobs, pool):
+    """A step limit of 17 inside 30 steps of 10-step graph replays: the job
+    stops at step 17 exactly, bitwise where a 17-step run alone ends, while
+    the job beside it runs all 30; a single engine stops alike."""
+    (ta, pa), (tb, pb) = jobs[0], jobs[1]
+    pool.admit(0, ta, pa)
+    pool.admit(1, tb, pb)
+    pool.set_step_limit(0, 17)
+    pool.step(30, 10)
+    assert pool.stats(0)["steps_done"] == 17 and pool.stats(1)["steps_done"] == 30
+    xa, va, _ = alone(pm, ta, pa, 17)
+    assert np.array_equal(pool.get_positions(0), xa) and np.array_equal(pool.get_velocities(0), va)
+    pool.retire(0)
+    pool.retire(1)
+    with pm.FusedEngine(ta, pa) as eng:
+        eng.set_step_limit(25)
+        eng.step(40)
+        assert eng.stats()["steps_done"] == 25
+        eng.set_step_limit(0)
+        eng.step(5)
+        assert eng.stats()["steps_done"] == 30
+
+
+def test_device_step_latency(pm, jobs, pool):
+    """Per-step latency from the device's step-end stamps: one sample per
+    step after a job's first, each the time since the previous step's end;
+    an admission between two ticks shows up as a long interval of the job
+    that kept running (the admission's device and host work sit between its
+    two steps)."""
+    import time
+    (ta, pa), (tb, pb) = jobs[0], jobs[1]
+    pool.admit(0, ta, pa)
+    t0 = time.perf_counter()
+    pool.step(40, 10)
+    host_ms = (time.perf_counter() - t0) * 1e3 / 40
+    lat = pool.latency_samples(0)
+    print(f"[latency] {len(lat)} samples, p50 {np.median(lat):.4f} ms, host {host_ms:.4f} ms/step", flush=True)
+    assert len(lat) == 39 and (lat > 0).all()
+    assert np.median(lat) <= 2.0 * host_ms
+    pool.admit(1, tb, pb)  # lowering, upload, re-sort and forces of the new job, between two ticks
+    pool.step(10, 10)
+    lat = pool.latency_samples(0)
+    gap = lat[39]  # the first step after the admission
+    print(f"[latency] step after the admission {gap:.4f} ms vs median {np.median(lat[:39]):.4f} ms", flush=True)
+    assert len(lat) == 49 and gap > 5.0 * np.median(lat[:39])
+    q = pool.latency(0)
+    assert q["samples"] == 49 and q["p99_ms"] >= q["p50_ms"] > 0
+    assert len(pool.latency_samples(1)) == 9
+    pool.retire(0)
+    pool.retire(1)
 
 
 def test_faulted_job_halts_alone(pm, jobs, pool):

@@ -6,16 +6,22 @@ from __future__ import annotations
 
 import types
 
+import numpy as np
 import pytest
 
 from prism_md.serving import Job, Scheduler, plan_slots
 
 
 class FakePool:
+    """Slots that count steps and stop at their step limit, as the device
+    does; each step's latency is 0.25 ms."""
+
     def __init__(self, atom_caps):
         self.caps = [types.SimpleNamespace(atoms=a) for a in atom_caps]
         self.log = []
         self.occupied = {}
+        self.steps = {}
+        self.limit = {}
 
     def fits(self, slot, top):
         return top.n_atoms < self.caps[slot].atoms
@@ -23,14 +29,25 @@ class FakePool:
     def admit(self, slot, top, params):
         assert slot not in self.occupied
         self.occupied[slot] = top
+        self.steps[slot] = 0
+        self.limit[slot] = 0
         self.log.append(("admit", slot, top.n_atoms))
+
+    def set_step_limit(self, slot, end_step):
+        self.limit[slot] = end_step
 
     def retire(self, slot):
         del self.occupied[slot]
-        self.log.append(("retire", slot))
+        self.log.append(("retire", slot, self.steps[slot]))
 
     def step(self, n, graph_steps):
         self.log.append(("step", n, tuple(sorted(self.occupied))))
+        for slot in self.occupied:
+            cap = self.limit[slot] or float("inf")
+            self.steps[slot] = int(min(cap, self.steps[slot] + n))
+
+    def latency_samples(self, slot):
+        return np.full(max(0, self.steps[slot] - 1), 0.25, np.float32)
 
     def get_positions(self, slot):
         return None
@@ -85,8 +102,25 @@ def test_smallest_fitting_slot_fifo_and_reuse():
     retires = [e for e in pool.log if e[0] == "retire"]
     assert len(admits) == len(retires) == 4
     rep = sch.report(wall=1.0)
-    assert rep["jobs_done"] == 4 and rep["step_latency_ms"]["p50"] > 0
+    assert rep["jobs_done"] == 4 and rep["step_latency_ms"]["p50"] == pytest.approx(0.25)
+    assert rep["step_latency_ms"]["samples"] == sum(j.steps - 1 for j in sch.done)
     assert rep["atom_steps_per_s"] == pytest.approx(sum(j.n_atoms * j.done_steps for j in sch.done))
+
+
+def test_jobs_end_at_their_exact_length():
+    """A job whose length is not a multiple of the replay length stops inside
+    a replay (device step limit) and is counted for exactly its steps."""
+    pool = FakePool([5000, 5000])
+    sch = Scheduler(pool, graph_steps=10, clock=Clock())
+    a, b = Job(0, top(1000), None, 25), Job(1, top(2000), None, 40)
+    sch.submit(a)
+    sch.submit(b)
+    while sch.tick():
+        pass
+    retired = {e[1]: e[2] for e in pool.log if e[0] == "retire"}
+    assert retired[a.slot] == 25 and retired[b.slot] == 40
+    assert a.done_steps == 25 and b.done_steps == 40
+    assert sch.atom_steps == 1000 * 25 + 2000 * 40
 
 
 def test_oversized_job_waits_without_blocking_others():
@@ -110,22 +144,22 @@ class FaultyPool(FakePool):
 
     def admit(self, slot, top, params):
         super().admit(slot, top, params)
-        self.steps[slot] = 0
         self.status[slot] = 0
 
     def step(self, n, graph_steps):
-        super().step(n, graph_steps)
+        self.log.append(("step", n, tuple(sorted(self.occupied))))
         err = None
         for slot in self.occupied:
             if self.status[slot]:
                 continue
             at = self.fault_at.get(slot)
+            cap = self.limit[slot] or float("inf")
             if at is not None and self.steps[slot] + n >= at:
                 self.steps[slot] = at
                 self.status[slot] = 1
                 err = err or RuntimeError(f"constraint iteration did not converge (system {slot})")
             else:
-                self.steps[slot] += n
+                self.steps[slot] = int(min(cap, self.steps[slot] + n))
         if err is not None:
             raise err
 
