@@ -33,9 +33,10 @@ One JSON line on stdout:
   nve          energy drift of the 2,000-atom chain over 100,000 steps without
                a thermostat, at 1 fs and 0.5 fs.
 
-The ensemble (400 steps per system instead of 10,000) and serving (60 s
-instead of 30 min) legs are shortened so the default run stays within
-minutes; --ensemble-steps and --serving-seconds run them longer.
+The ensemble runs its full 10,000 steps per system (its one-at-a-time
+baseline is timed on 1,000 steps per system); the serving leg runs 150 s
+instead of 30 min so the default run stays within minutes
+(--serving-seconds runs it longer).
   cpu_baseline.small_chain_full  the 2,000-atom chain, 1,000 steps at 2 fs,
                through the FP64 oracle in full.
 """
@@ -129,14 +130,19 @@ def nve_leg(pm, n_atoms, steps, dt_ps, every, seed=7, minimize=2000):
                     fault = {"after_steps": (k - 1) * every, "error": str(ex)}
                     break
             en = eng.energies()
+            if not np.isfinite(en["potential"] + en["kinetic"]):  # an overflowing pair energy: blown up
+                fault = {"after_steps": k * every, "error": "non-finite total energy"}
+                break
             ts.append(k * every * dt_ps)
             es.append(en["potential"] + en["kinetic"])
             ks.append(en["kinetic"])
         st = eng.stats()
+    if len(es) < 3:
+        return {"atoms": n_atoms, "steps": 0, "dt_ps": dt_ps, "fault": fault, "wall_s": round(time.perf_counter() - t0, 1)}
     t = np.asarray(ts)
     e = np.asarray(es)
     slope = float(np.polyfit(t, e, 1)[0])
-    return {"atoms": n_atoms, "steps": steps, "dt_ps": dt_ps, "sample_every": every,
+    return {"atoms": n_atoms, "steps": int(round(t[-1] / dt_ps)), "dt_ps": dt_ps, "sample_every": every,
             "baseline": "after the first step (velocities projected onto the constraints)",
             "drift_kcal_mol_per_ps": round(slope, 4),
             "drift_kcal_mol_per_ns_per_atom": round(slope * 1e3 / n_atoms, 5),
@@ -212,33 +218,39 @@ def copy_calibration(mib: int, reps: int = 5):
 
 
 def ensemble_leg(pm, n_systems, steps, graph_steps, tune_buckets, chunk_tiles=0, plain_runs=True,
-                 seed=0x50524953 + 3):
+                 seed=0x50524953 + 3, plain_steps=1000):
     """Batched ensemble (one launch per phase over all systems, captured
     hipGraph) against the same systems run one at a time on the plain path.
     Sizes log-uniform in [500, 20000] atoms, temperatures uniform in
     [280, 340] K, 4 fs, per-system generator and noise seeds.  The sort-cell
-    size and re-sort interval of each system come from the autotuner's
-    choice for its size bucket (prism_md.autotune).  The batch's nonbonded
-    units hold chunk_tiles row tiles (a batch has many units, so long units
-    win); the one-at-a-time baseline keeps the engine's per-system default,
-    which is the faster choice for a system alone."""
+    size, re-sort interval and row tiles per nonbonded unit of each system
+    come from the autotuner's choice for its size bucket (prism_md.autotune;
+    chunk_tiles > 0 overrides the unit length for every system); the
+    one-at-a-time baseline keeps the engine's per-system unit length, the
+    faster choice for a system alone.  The batch runs all `steps` steps; the
+    one-at-a-time baseline times `plain_steps` steps of each system after
+    the same warm-up (its per-step time does not drift, and the full length
+    one at a time would take minutes)."""
     import numpy as np
     from prism_md import autotune
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(500), np.log(20000), n_systems)).astype(int)
     temps = rng.uniform(280.0, 340.0, n_systems)
     t0 = time.perf_counter()
-    table = autotune.tune(pm, autotune.log_buckets(500, 20000, tune_buckets), graph_steps=graph_steps) \
+    table = autotune.tune(pm, autotune.log_buckets(500, 20000, tune_buckets), graph_steps=graph_steps,
+                          progress=lambda e: log(f"autotune bucket {e['lo']}-{e['hi']}: cell {e['sort_cell_A']} "
+                                                 f"resort {e['resort_every']} tiles {e['chunk_tiles']} "
+                                                 f"{e['us_per_step']} us/step")) \
         if tune_buckets > 0 else []
-    log(f"autotune: {[(e['lo'], e['hi'], e['sort_cell_A'], e['resort_every'], e['us_per_step']) for e in table]} "
+    log(f"autotune: {[(e['lo'], e['hi'], e['sort_cell_A'], e['resort_every'], e['chunk_tiles'], e['us_per_step']) for e in table]} "
         f"in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     tops = [pm.Topology.generate(int(n), seed=seed + 1 + k) for k, n in enumerate(sizes)]
     kws = [dict(seed=seed + 1000 + k, stream_id=k, temperature_K=float(T), end_temperature_K=float(T),
                 **(autotune.choice_for(table, int(n)) if table else {}))
            for k, (T, n) in enumerate(zip(temps, sizes))]
-    params = [pm.default_params(chunk_tiles=chunk_tiles, **kw) for kw in kws]
-    plain_params = [pm.default_params(**kw) for kw in kws]
+    params = [pm.default_params(**dict(kw, **({"chunk_tiles": chunk_tiles} if chunk_tiles else {}))) for kw in kws]
+    plain_params = [pm.default_params(**{k: v for k, v in kw.items() if k != "chunk_tiles"}) for kw in kws]
     total = int(sizes.sum())
     log(f"ensemble: {n_systems} systems, {total} atoms generated in {time.perf_counter() - t0:.1f} s")
     ens = pm.Ensemble(tops, params)
@@ -275,10 +287,11 @@ def ensemble_leg(pm, n_systems, steps, graph_steps, tune_buckets, chunk_tiles=0,
     ens.close()
     batched = atom_steps / wall_b
     # plain path: the same systems one at a time, same starting positions
-    # (systems that faulted in the batch fault at the same step alone: skipped)
+    # (systems that faulted in the batch are skipped), plain_steps steps each
     wall_p = 0.0
     plain_lat = []
     plain_atoms = 0
+    n_plain = min(plain_steps, done_b)
     for k in range(n_systems if plain_runs else 0):
         if k in bad:
             continue
@@ -287,20 +300,22 @@ def ensemble_leg(pm, n_systems, steps, graph_steps, tune_buckets, chunk_tiles=0,
             eng.set_state(x=starts[k])
             eng.step(2 * graph_steps)
             t1 = time.perf_counter()
-            eng.step(done_b)
+            eng.step(n_plain)
             dt = time.perf_counter() - t1
             wall_p += dt
-            plain_lat.append(dt / done_b)
-    plain = plain_atoms * done_b / wall_p if wall_p > 0 else None
+            plain_lat.append(dt / n_plain)
+    plain = plain_atoms * n_plain / wall_p if wall_p > 0 else None
     lat = np.array(lat) * 1e3
     return {"n_systems": n_systems, "atoms": total, "steps": done_b, "graph_steps": graph_steps,
             "atom_steps_per_s": round(batched, 1), "plain_atom_steps_per_s": round(plain, 1) if plain else None,
-            "speedup_vs_plain": round(batched / plain, 3) if plain else None, "chunk_tiles": chunk_tiles,
+            "speedup_vs_plain": round(batched / plain, 3) if plain else None,
+            "chunk_tiles": chunk_tiles or "autotuned per bucket", "plain_steps_per_system": n_plain,
             "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                                 "p99": round(float(np.percentile(lat, 99)), 4)},
             "plain_step_latency_ms_all_systems_sequential": round(float(sum(plain_lat)) * 1e3, 4) if plain_lat else None,
             "systems_with_faults": bad,
-            "autotune": [{k: e[k] for k in ("lo", "hi", "sort_cell_A", "resort_every", "us_per_step")} for e in table],
+            "autotune": [{k: e[k] for k in ("lo", "hi", "sort_cell_A", "resort_every", "chunk_tiles", "us_per_step")}
+                         for e in table],
             "sizes": "log-uniform 500-20000 atoms", "temperatures_K": "uniform 280-340"}
 
 
@@ -373,6 +388,14 @@ def serving_leg(pm, seconds, rate, budget_gib, graph_steps, seed=0x50524953 + 5)
     wall = time.perf_counter() - t0
     stop.set()
     rep = sch.report(wall)
+    # steady state: completions keep pace with arrivals in the second half
+    half = wall / 2.0
+    done_t = [j.t_done - t0 for j in sch.done + sch.failed if j.t_done is not None]
+    arr_t = [a for a in arrivals if a <= wall]
+    rep["arrivals_first_half"] = int(sum(a < half for a in arr_t))
+    rep["arrivals_second_half"] = int(sum(a >= half for a in arr_t))
+    rep["completions_first_half"] = int(sum(t < half for t in done_t))
+    rep["completions_second_half"] = int(sum(t >= half for t in done_t))
     rep.update({"seconds": round(wall, 1), "arrival_rate_per_s": rate, "jobs_submitted": submitted,
                 "hbm_budget_gib": budget_gib, "pool_bytes": pool.device_bytes(),
                 "slots": {str(c): n for c, n in zip(classes, counts)},
@@ -412,16 +435,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--sample-every", type=int, default=10, help="nonbonded timing sample interval (steps)")
     ap.add_argument("--ensemble-systems", type=int, default=256, help="batched ensemble size (0 = skip)")
-    ap.add_argument("--ensemble-steps", type=int, default=400)
+    ap.add_argument("--ensemble-steps", type=int, default=10000, help="steps of every system (the workload's length)")
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--tune-buckets", type=int, default=4, help="autotuner size buckets for the ensemble (0 = defaults)")
     ap.add_argument("--ensemble-no-plain", action="store_true",
                     help="skip the one-at-a-time baseline of the ensemble (for profiling the batched steps)")
-    ap.add_argument("--ensemble-chunk-tiles", type=int, default=16,
-                    help="row tiles per nonbonded work unit in the batched ensemble (0 = the engine's choice per "
-                         "system; 16 measured 3 %% faster than that for the 256-system batch)")
-    ap.add_argument("--serving-seconds", type=float, default=60.0, help="serving loop run time (0 = skip)")
-    ap.add_argument("--serving-rate", type=float, default=1.0, help="job arrivals per second")
+    ap.add_argument("--ensemble-chunk-tiles", type=int, default=0,
+                    help="row tiles per nonbonded work unit in the batched ensemble (0 = the autotuner's choice per "
+                         "size bucket)")
+    ap.add_argument("--serving-seconds", type=float, default=150.0, help="serving loop run time (0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=0.25,
+                    help="job arrivals per second (below the pool's throughput, so the loop reaches a steady state)")
     ap.add_argument("--serving-budget-gib", type=float, default=16.0)
     ap.add_argument("--nve-steps", type=int, default=100000, help="NVE drift leg length (0 = skip)")
     ap.add_argument("--nve-dt", type=float, default=0.001, help="NVE drift leg timestep (ps)")

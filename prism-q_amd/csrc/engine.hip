@@ -132,7 +132,7 @@ int Engine::d2h(void *dst, const void *src, size_t bytes) {
 Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
     for (hipEvent_t e : ev_pool_) hipEventDestroy(e);
-    if (graph_) hipGraphExecDestroy(graph_);
+    for (auto &g : graphs_) hipGraphExecDestroy(g.second);
     for (void *p : allocs_) hipFree(p);
     if (own_stream_ && stream_) hipStreamDestroy(stream_);
 }
@@ -1051,23 +1051,42 @@ int Engine::compute_forces(unsigned mask, int s, float *F_out, int64_t *F_fixed,
     return check_status();
 }
 
+// Steps without explicit noise replay captured graphs of whole steps.  With
+// nonbonded sampling on (kernel_timing), each sampling period is one eager
+// step whose nonbonded launch is bracketed by events, then a graph of the
+// period's other steps.  Explicit noise is staged per step, so those steps
+// run eagerly.
 int Engine::step(uint64_t n_steps, const float *noise) {
     if (noise && sys_.size() != 1) return fail(PRISM_MD_E_UNSUPPORTED, "explicit noise needs a single system");
-    const int n = noise ? sys_[0].n_atoms : 0;
+    if (!noise) {
+        if (timing_every_ <= 0) {
+            if (int rc = replay(n_steps, SINGLE_GRAPH_STEPS)) return rc;
+        } else {
+            const uint64_t period = (uint64_t)timing_every_;
+            for (uint64_t done = 0; done < n_steps;) {
+                launch_step(nullptr, nullptr);  // sampled
+                const uint64_t rest = std::min<uint64_t>(period - 1, n_steps - done - 1);
+                if (rest)
+                    if (int rc = replay(rest, (int)(period - 1))) return rc;
+                done += 1 + rest;
+            }
+        }
+        host_steps_ += n_steps;
+        if (int rc = hipcheck(hipGetLastError(), "step launch")) return rc;
+        const int rc = check_status();
+        harvest_timing();
+        return rc;
+    }
+    const int n = sys_[0].n_atoms;
     const size_t per_half = 3 * (size_t)n;
     for (uint64_t k = 0; k < n_steps; ++k) {
-        const float *nn = nullptr, *np = nullptr;
-        if (noise) {
-            if (int rc = hipcheck(hipMemcpyAsync(d_noise_, noise + (size_t)k * 2 * per_half, 2 * per_half * sizeof(float),
-                                                 hipMemcpyHostToDevice, stream_),
-                                  "noise upload"))
-                return rc;
-            nn = d_noise_;
-            np = d_noise_ + per_half;
-        }
-        launch_step(nn, np);
-        if (noise)  // the staging buffer is reused next step
-            if (int rc = hipcheck(hipStreamSynchronize(stream_), "noise sync")) return rc;
+        if (int rc = hipcheck(hipMemcpyAsync(d_noise_, noise + (size_t)k * 2 * per_half, 2 * per_half * sizeof(float),
+                                             hipMemcpyHostToDevice, stream_),
+                              "noise upload"))
+            return rc;
+        launch_step(d_noise_, d_noise_ + per_half);
+        // the staging buffer is reused next step
+        if (int rc = hipcheck(hipStreamSynchronize(stream_), "noise sync")) return rc;
     }
     host_steps_ += n_steps;
     if (int rc = hipcheck(hipGetLastError(), "step launch")) return rc;
@@ -1108,32 +1127,50 @@ int Engine::kernel_timing(int every, double *mean_ms, uint64_t *samples) {
     return PRISM_MD_OK;
 }
 
-int Engine::step_graph(uint64_t n_steps, int graph_steps) {
+// n_steps steps as replays of a captured graph of graph_steps steps (captured
+// once per length and kept), the remainder eagerly.  No event records are
+// captured.
+int Engine::replay(uint64_t n_steps, int graph_steps) {
     if (graph_steps < 1) return fail(PRISM_MD_E_INVALID, "graph_steps");
-    if (!graph_ || graph_steps_ != graph_steps) {
-        if (graph_) hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : graphs_)
+        if (g.first == graph_steps) exec = g.second;
+    if (!exec && n_steps >= (uint64_t)graph_steps) {
+        if (graphs_.size() >= MAX_GRAPHS) {  // keep the cache small: drop the oldest
+            hipGraphExecDestroy(graphs_.front().second);
+            graphs_.erase(graphs_.begin());
+        }
         hipGraph_t g;
         const int every = timing_every_;
-        const uint64_t evals = force_evals_;
+        const uint64_t evals = force_evals_;  // capturing evaluates nothing
         timing_every_ = 0;  // no event records inside the captured graph
-        if (int rc = hipcheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture")) return rc;
+        if (int rc = hipcheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture")) {
+            timing_every_ = every;
+            return rc;
+        }
         for (int k = 0; k < graph_steps; ++k) launch_step(nullptr, nullptr);
         const hipError_t ce = hipStreamEndCapture(stream_, &g);
         timing_every_ = every;
         force_evals_ = evals;
         if (int rc = hipcheck(ce, "end capture")) return rc;
-        const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (int rc = hipcheck(e, "instantiate")) return rc;
-        graph_steps_ = graph_steps;
+        graphs_.emplace_back(graph_steps, exec);
     }
     uint64_t done = 0;
-    for (; done + graph_steps <= n_steps; done += graph_steps)
-        if (int rc = hipcheck(hipGraphLaunch(graph_, stream_), "graph launch")) return rc;
-    for (; done < n_steps; ++done) launch_step(nullptr, nullptr);
+    if (exec)
+        for (; done + graph_steps <= n_steps; done += graph_steps)
+            if (int rc = hipcheck(hipGraphLaunch(exec, stream_), "graph launch")) return rc;
+    force_evals_ += done;
+    for (; done < n_steps; ++done) launch_step(nullptr, nullptr);  // counts its own evaluation
+    return PRISM_MD_OK;
+}
+
+int Engine::step_graph(uint64_t n_steps, int graph_steps) {
+    if (graph_steps < 1) return fail(PRISM_MD_E_INVALID, "graph_steps");
+    if (int rc = replay(n_steps, graph_steps)) return rc;
     host_steps_ += n_steps;
-    force_evals_ += n_steps;
     if (int rc = hipcheck(hipGetLastError(), "graph step")) return rc;
     return check_status();
 }

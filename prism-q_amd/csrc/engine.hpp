@@ -178,8 +178,11 @@ class Engine {
     double *d_fire_ = nullptr;    // FIRE partials
     SysDesc *d_desc_ = nullptr;
     void *static_staging_ = nullptr;  // this device's address of the static pair staging
-    hipGraphExec_t graph_ = nullptr;
-    int graph_steps_ = 0;
+    // captured step graphs by length (at most MAX_GRAPHS kept)
+    std::vector<std::pair<int, hipGraphExec_t>> graphs_;
+    static constexpr size_t MAX_GRAPHS = 4;
+    static constexpr int SINGLE_GRAPH_STEPS = 20;  // graph length of prism_md_step
+    int replay(uint64_t n_steps, int graph_steps);
     uint64_t host_steps_ = 0;
     uint64_t force_evals_ = 0;
 

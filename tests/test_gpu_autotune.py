@@ -35,8 +35,8 @@ def test_layout_parameters_do_not_change_forces(pm, oracle):
 def test_tuner_on_device(pm):
     from prism_md import autotune
     table = autotune.tune(pm, [(800, 1200)], steps=60, graph_steps=10, copies=2,
-                          sort_cells=(1.5, 3.0), resort_every=(4, 16))
+                          sort_cells=(1.5, 3.0), resort_every=(4, 16), chunk_tiles=(8, 24))
     e = table[0]
-    assert e["sort_cell_A"] in (1.5, 3.0) and e["resort_every"] in (4, 16)
-    assert len(e["trials"]) == 4 and all(t["us_per_step"] > 0 for t in e["trials"])
-    assert e["us_per_step"] == min(t["us_per_step"] for t in e["trials"])
+    assert e["sort_cell_A"] in (1.5, 3.0) and e["resort_every"] in (4, 16) and e["chunk_tiles"] in (8, 24)
+    assert len(e["trials"]) == 6 and all(t["us_per_step"] > 0 for t in e["trials"])
+    assert e["us_per_step"] == min(t["us_per_step"] for t in e["trials"][4:])
