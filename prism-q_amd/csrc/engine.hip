@@ -646,7 +646,7 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     // tiles per nonbonded unit: short units balance the persistent grid when
     // units are few, long ones save per-unit start-up when they are many
     // (measured best: 10 at 25k atoms, 16 at 250k)
-    D.chunk = p.chunk_tiles > 0 ? p.chunk_tiles : (t->n_atoms < 100000 ? 10 : 16);
+    D.chunk = p.chunk_tiles > 0 ? p.chunk_tiles : (t->n_atoms < 100000 ? 10 : 24);
     D.dt = (float)p.dt_ps;
     D.cutoff = (float)p.cutoff_A;
     D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);

@@ -17,6 +17,7 @@ namespace prism {
 namespace {
 
 constexpr int NB_WAVES = 4;
+constexpr int RING = 128;  // survivor ring per wave: two batches of 64
 
 // Cutoff test of the nonbonded pair: r2 = fma(dz, dz, fma(dy, dy, dx*dx)) in
 // FP32 with d = x_i - x_j.  The oracle decides pair membership with the same
@@ -95,10 +96,16 @@ template <bool ENERGY, bool DIRECT, bool TM>
 __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     const int wave = wave_id(), lane = threadIdx.x & 63;
     const int il = lane & (SUB - 1), js = lane >> 4;
+    // per wave: the survivors of a masked tile, and a ring of the survivors of
+    // unmasked tiles evaluated 64 at a time
     __shared__ float4 jpos_s[NB_WAVES][64];
     __shared__ float4 jpar_s[NB_WAVES][64];
+    __shared__ float4 rpos_s[NB_WAVES][RING];
+    __shared__ float4 rpar_s[NB_WAVES][RING];
     float4 *jpos = jpos_s[wave];
     float4 *jpar = jpar_s[wave];
+    float4 *rpos = rpos_s[wave];
+    float4 *rpar = rpar_s[wave];
     const int n_units = P.counters->n_units;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // Workgroups are dispatched round-robin over the XCDs (workgroup w on XCD
@@ -157,10 +164,12 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 lo[0] = lo[1] = lo[2] = 3.0e38f;
                 hi[0] = hi[1] = hi[2] = -3.0e38f;
             }
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = wave_min(lo[k]);
-                hi[k] = wave_max(hi[k]);
-            }
+            // the four 16-lane rows hold the same 16 atoms: reduce within a row
+            for (int k = 0; k < 3; ++k)
+                for (int o = 1; o < SUB; o <<= 1) {
+                    lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
+                    hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
+                }
             const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
             const float hx = 0.5f * (hi[0] - lo[0]), hy = 0.5f * (hi[1] - lo[1]), hz = 0.5f * (hi[2] - lo[2]);
 
@@ -171,6 +180,11 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
             float4 xj = P.xq[jb * BLOCK + lane];
             float2 lj = P.lj[jb * BLOCK + lane];
             float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
+            // The ring holds `queued` survivors from `head` (0 or RING/2) on;
+            // a full batch is always the contiguous [head, head + RING/2), and
+            // appends wrap at most once (queued < RING/2 before an append of
+            // at most 64).  Wave-uniform.
+            int head = 0, queued = 0;
             for (int k = 0; k < n_t; ++k) {
                 const int ms = __shfl(my_ms, k);
                 float4 xn = xj;
@@ -186,7 +200,8 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 const bool active = ex * ex + ey * ey + ez * ez <= reach2;
                 const unsigned long long m = __ballot(active);
                 const int n_act = __popcll(m);
-                if (n_act > 0) {
+                if (ms >= 0 && n_act > 0) {
+                    // a tile with skipped pairs: its survivors now, with their mask word
                     if (active) {
                         const int pos = __popcll(m & lt);
                         jpos[pos] = xj;
@@ -198,17 +213,39 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                         jpar[lane] = make_float4(0.0f, 0.0f, __int_as_float(0), 0.0f);
                     }
                     wave_lds_fence();
-                    const int n_iter = n_pad >> 2;
-                    if (ms >= 0) {
-                        const unsigned long long excl = P.masks[((size_t)b * MAX_MASKS + ms) * 64 + q * SUB + il];
-                        tile_pairs<ENERGY, true>(xi, li, n_iter, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
-                    } else {
-                        tile_pairs<ENERGY, false>(xi, li, n_iter, js, jpos, jpar, 0ull, lo_bits, span, inv_rc, inv_rc2, fx, fy, fz, e);
-                    }
+                    const unsigned long long excl = P.masks[((size_t)b * MAX_MASKS + ms) * 64 + q * SUB + il];
+                    tile_pairs<ENERGY, true>(xi, li, n_pad >> 2, js, jpos, jpar, excl, lo_bits, span, inv_rc, inv_rc2,
+                                             fx, fy, fz, e);
                     wave_lds_fence();
+                } else if (n_act > 0) {
+                    if (active) {
+                        const int pos = (head + queued + __popcll(m & lt)) & (RING - 1);
+                        rpos[pos] = xj;
+                        rpar[pos] = make_float4(lj.x, lj.y, 0.0f, 0.0f);
+                    }
+                    queued += n_act;
+                    if (queued >= RING / 2) {  // a full batch: 16 iterations of 16 i x 4 j
+                        wave_lds_fence();
+                        tile_pairs<ENERGY, false>(xi, li, RING / 8, js, rpos + head, rpar + head, 0ull, lo_bits, span,
+                                                  inv_rc, inv_rc2, fx, fy, fz, e);
+                        wave_lds_fence();  // read before the slots are written again
+                        head ^= RING / 2;
+                        queued -= RING / 2;
+                    }
                 }
                 xj = xn;
                 lj = ln;
+            }
+            if (queued > 0) {  // the unit's last, partial batch, padded to whole iterations
+                const int n_pad = (queued + 3) & ~3;
+                if (lane >= queued && lane < n_pad) {
+                    rpos[head + lane] = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f);
+                    rpar[head + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                wave_lds_fence();
+                tile_pairs<ENERGY, false>(xi, li, n_pad >> 2, js, rpos + head, rpar + head, 0ull, lo_bits, span, inv_rc,
+                                          inv_rc2, fx, fy, fz, e);
+                wave_lds_fence();
             }
             // sum the four j-slot partials of each i-atom (lanes il, il+16, il+32,
             // il+48) in a fixed order and store the unit's partial; the gather
