@@ -1431,13 +1431,13 @@ int Engine::set_active(int s, bool active) {
                           "activate"))
         return rc;
     if (int rc = upload_term_live()) return rc;
+    EngineCounters c{};  // source of an asynchronous copy: lives until the synchronisation below
     if (active) {
         // re-sort and rebuild this system, re-claim the unit list, and give it
         // its carried force before its first step
         if (int rc = refresh(s)) return rc;
     } else {
         // drop its units: every active block re-claims, no row changes
-        EngineCounters c{};
         c.any_dirty = 1;
         // the flags only: the step-end count behind them keeps running
         if (int rc = hipcheck(hipMemcpyAsync(P_.counters, &c, offsetof(EngineCounters, pad), hipMemcpyHostToDevice, stream_),

@@ -1,8 +1,10 @@
 // kernels_force.hip -- force evaluation: nonbonded cluster-pair tiles,
-// bonded terms and exact 1-4 exceptions.  Every work item stores its partial
-// forces to its own location; a gather sums each atom's partials in a fixed
-// order into int64 fixed point (2^24 per kcal/mol/A), so the force is
-// bitwise reproducible and no device-scope atomics sit on the hot path.
+// bonded terms and exact 1-4 exceptions.  Every partial force is converted to
+// int64 fixed point (2^24 per kcal/mol/A) before it is summed, so the total is
+// independent of summation order and bitwise reproducible: the MD step's
+// evaluation adds each work unit's partial into the atom's accumulator with an
+// integer atomic; the evaluations outside the step store per-unit partials
+// and a gather sums them per atom in a fixed order.
 //
 // Replaces the FORCE_N / FORCE_NP1 gather of prism_amber_phase_body_v1
 // (nhs_amber_fused.cu:2981-3484) with its term functions compute_bond_force
