@@ -824,10 +824,11 @@ int Engine::refresh(int only) {
 void Engine::launch_forces(bool step) {
     const bool sample = timing_every_ > 0 && timing_counter_++ % (uint64_t)timing_every_ == 0 &&
                         2 * ev_used_ + 1 < ev_pool_.size();
-    if (sample) hipEventRecord(ev_pool_[2 * ev_used_], stream_);
-    if (step) launch_nonbonded_direct(P_, stream_);
-    else launch_nonbonded(P_, false, stream_);
-    if (sample) hipEventRecord(ev_pool_[2 * ev_used_++ + 1], stream_);
+    // a sampled launch carries its own start / stop events
+    hipEvent_t start = sample ? ev_pool_[2 * ev_used_] : nullptr, stop = sample ? ev_pool_[2 * ev_used_ + 1] : nullptr;
+    if (sample) ++ev_used_;
+    if (step) launch_nonbonded_direct(P_, stream_, start, stop);
+    else launch_nonbonded(P_, false, stream_, start, stop);
     if (!step) {
         launch_bonded(P_, PRISM_MD_TERM_ALL, false, stream_);
         launch_gather(P_, PRISM_MD_TERM_ALL, false, stream_);

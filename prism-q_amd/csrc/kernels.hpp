@@ -104,11 +104,14 @@ void launch_rows(const DevPtrs &P, hipStream_t s);
 void launch_consume(const DevPtrs &P, int step_offset, hipStream_t s);
 // slot_of / xq / lj of system s's n_slots sorted slots from its uploaded perm
 void launch_restore_slots(const DevPtrs &P, int s, int n_slots, hipStream_t st);
-void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s);
+// start / stop: optional HIP events recorded with the launch, bracketing the
+// kernel's execution (sampled timing)
+void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s, hipEvent_t start = nullptr,
+                      hipEvent_t stop = nullptr);
 // The MD step's evaluation: nonbonded partials and the bonded contribution
 // slots (written beforehand, see launch_sort) added straight into the force
 // accumulators, which k_first_half zeroed; no gather follows.
-void launch_nonbonded_direct(const DevPtrs &P, hipStream_t s);
+void launch_nonbonded_direct(const DevPtrs &P, hipStream_t s, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);
 void launch_second_half(const DevPtrs &P, const float *noise, hipStream_t s);

@@ -11,6 +11,8 @@
 // (:354-386), compute_angle_force (:388-448), compute_dihedral_force
 // (:524-625), compute_nonbonded_force (:643-717) and
 // ordinary_cutoff_potential_offset_v1 (:631-640).
+#include <hip/hip_ext.h>
+
 #include "bonded.hpp"
 #include "device_common.hpp"
 
@@ -476,24 +478,32 @@ template <bool ENERGY, bool DIRECT, bool TM> static int resident_blocks() {
     return c;
 }
 
-template <bool ENERGY, bool DIRECT, bool TM> static void launch_nb(const DevPtrs &P, hipStream_t st) {
+// start / stop, when given, are HIP events recorded with the launch itself
+// (hipExtLaunchKernelGGL): they bracket the kernel's execution, not the gap
+// before its dispatch.
+template <bool ENERGY, bool DIRECT, bool TM>
+static void launch_nb(const DevPtrs &P, hipStream_t st, hipEvent_t start = nullptr, hipEvent_t stop = nullptr) {
     // persistent grid: every wave resident at once (no second round of
     // workgroups), but no more waves than a typical unit list needs
     const int want = (P.n_blocks_arena * QUARTERS * 12 + NB_WAVES - 1) / NB_WAVES;
     const int res = resident_blocks<ENERGY, DIRECT, TM>();
     int grid = max(1, min(want, res));
     if (res % N_XCD == 0) grid = (grid + N_XCD - 1) / N_XCD * N_XCD;  // whole XCD rounds
-    hipLaunchKernelGGL((k_nonbonded<ENERGY, DIRECT, TM>), dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
+    if (start || stop)
+        hipExtLaunchKernelGGL((k_nonbonded<ENERGY, DIRECT, TM>), dim3(grid), dim3(64 * NB_WAVES), 0, st, start, stop,
+                              0, P);
+    else
+        hipLaunchKernelGGL((k_nonbonded<ENERGY, DIRECT, TM>), dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
 }
 
-void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st) {
-    if (energy) launch_nb<true, false, false>(P, st);  // stores unit partials: no bonded slots read
-    else launch_nb<false, false, false>(P, st);
+void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
+    if (energy) launch_nb<true, false, false>(P, st, start, stop);  // stores unit partials: no bonded slots read
+    else launch_nb<false, false, false>(P, st, start, stop);
 }
 
-void launch_nonbonded_direct(const DevPtrs &P, hipStream_t st) {
-    if (P.term_major) launch_nb<false, true, true>(P, st);
-    else launch_nb<false, true, false>(P, st);
+void launch_nonbonded_direct(const DevPtrs &P, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
+    if (P.term_major) launch_nb<false, true, true>(P, st, start, stop);
+    else launch_nb<false, true, false>(P, st, start, stop);
 }
 
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t st) {
