@@ -157,3 +157,19 @@ def test_pair_on_the_list_edge_is_kept(pm):
     with pm.FusedEngine(make_topology(pm, x)) as eng:
         for mode in (0, 1):
             assert eng.pairs(mode).tolist() == [[0, 1]], mode
+
+
+def test_coordinate_beyond_1000_A_is_a_divergence_fault(pm):
+    """heartbeat_check (housekeeping.cu:27-45) flags a coordinate beyond
+    1000 A: an atom drifting out past it latches E_DIVERGED at the step that
+    takes it there, naming the atom, and the system halts there."""
+    import sys
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    from toy import make_topology
+    x = np.array([[999.5, 0.0, 0.0], [-20.0, 0.0, 0.0]], np.float32)
+    with pm.FusedEngine(make_topology(pm, x)) as eng:
+        eng.set_state(v=np.array([[200.0, 0.0, 0.0], [0.0, 0.0, 0.0]], np.float32))
+        with pytest.raises(pm.PrismMdError) as ex:
+            eng.step(3)
+        assert ex.value.rc == pm.E_DIVERGED and "atom 0" in str(ex.value) and "step 0" in str(ex.value)
+        assert eng.stats()["steps_done"] == 1
