@@ -821,9 +821,8 @@ int Engine::refresh(int only) {
 // bonded, then the fixed-order gather.  (Running the bonded terms on a forked
 // second stream beside the nonbonded kernel was measured slower: the fork and
 // join cost more than the overlap gains.)
-void Engine::launch_forces(bool step) {
-    const bool sample = timing_every_ > 0 && timing_counter_++ % (uint64_t)timing_every_ == 0 &&
-                        2 * ev_used_ + 1 < ev_pool_.size();
+void Engine::launch_forces(bool step, bool sample) {
+    sample = sample && 2 * ev_used_ + 1 < ev_pool_.size();
     // a sampled launch carries its own start / stop events
     hipEvent_t start = sample ? ev_pool_[2 * ev_used_] : nullptr, stop = sample ? ev_pool_[2 * ev_used_ + 1] : nullptr;
     if (sample) ++ev_used_;
@@ -836,7 +835,7 @@ void Engine::launch_forces(bool step) {
 }
 
 void Engine::full_force() {
-    launch_forces(false);
+    launch_forces(false, false);
     ++force_evals_;
 }
 
@@ -844,11 +843,11 @@ void Engine::full_force() {
 // kernel's launch (spare workgroups after the sort and refresh ones) and the
 // nonbonded kernel adds every contribution straight into the accumulators
 // the first half zeroed, so no bonded or gather kernel runs.
-void Engine::launch_step(const float *noise_n, const float *noise_np1) {
+void Engine::launch_step(const float *noise_n, const float *noise_np1, bool sample) {
     launch_first_half(P_, noise_n, stream_);
     launch_sort(P_, true, stream_);
     launch_rows(P_, stream_);
-    launch_forces(true);
+    launch_forces(true, sample);
     launch_second_half(P_, noise_np1, stream_);
     ++force_evals_;
 }
@@ -1065,7 +1064,7 @@ int Engine::step(uint64_t n_steps, const float *noise) {
         } else {
             const uint64_t period = (uint64_t)timing_every_;
             for (uint64_t done = 0; done < n_steps;) {
-                launch_step(nullptr, nullptr);  // sampled
+                launch_step(nullptr, nullptr, true);  // sampled
                 const uint64_t rest = std::min<uint64_t>(period - 1, n_steps - done - 1);
                 if (rest)
                     if (int rc = replay(rest, (int)(period - 1))) return rc;
@@ -1085,7 +1084,8 @@ int Engine::step(uint64_t n_steps, const float *noise) {
                                              hipMemcpyHostToDevice, stream_),
                               "noise upload"))
             return rc;
-        launch_step(d_noise_, d_noise_ + per_half);
+        // with sampling on, every timing_every-th step of the call is sampled
+        launch_step(d_noise_, d_noise_ + per_half, timing_every_ > 0 && k % (uint64_t)timing_every_ == 0);
         // the staging buffer is reused next step
         if (int rc = hipcheck(hipStreamSynchronize(stream_), "noise sync")) return rc;
     }
@@ -1117,7 +1117,6 @@ int Engine::kernel_timing(int every, double *mean_ms, uint64_t *samples) {
     if (every != timing_every_ || every == 0) {
         timed_ms_ = 0.0;
         timed_n_ = 0;
-        timing_counter_ = 0;
     }
     if (every > 0 && ev_pool_.empty()) {
         ev_pool_.resize(2048);

@@ -104,7 +104,7 @@ class Engine {
     int lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p);
     int upload_slot(int s);
     template <class T> int upload_range(T *d, const std::vector<T> &h, size_t begin, size_t count);
-    void launch_step(const float *noise_n, const float *noise_np1);
+    void launch_step(const float *noise_n, const float *noise_np1, bool sample = false);
     void full_force();
     void harvest_timing();
 
@@ -159,12 +159,14 @@ class Engine {
     DevPtrs P_{};
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
-    void launch_forces(bool step);  // nonbonded, bonded, gather; or the step's direct nonbonded only
+    // nonbonded, bonded, gather; or the step's direct nonbonded only; sample:
+    // bracket the nonbonded launch with the next pair of timing events
+    void launch_forces(bool step, bool sample);
     std::vector<void *> allocs_;
     int timing_every_ = 0;        // sampled nonbonded timing (kernel_timing)
     std::vector<hipEvent_t> ev_pool_;  // pairs: [2k] before, [2k+1] after
     size_t ev_used_ = 0;
-    uint64_t timing_counter_ = 0, timed_n_ = 0;
+    uint64_t timed_n_ = 0;
     double timed_ms_ = 0.0;
     bool alloc_failed_ = false;
     std::string err_;

@@ -61,22 +61,25 @@ def nve_start(pm):
 
 
 def test_nve_energy_conservation(pm, nve_start):
-    """gamma = 0 selects exact kicks (no force clamp).  At 0.5 fs the total
-    energy of the relaxed 2,000-atom chain stays within 1 % of the kinetic
-    energy over 5 ps, and the error is the time step's: doubling dt raises it
-    well beyond twofold, which a force error (independent of dt) would not
-    do.  The integrator's own error is second order (ratio ~4); the force
-    jump at the potential-shift cutoff adds a part that grows with dt too,
-    and the 5 ps samples of this small system scatter, so the ratio of the
-    mean |error| over the ten samples is bounded by 2.5 and 10."""
+    """gamma = 0 selects exact kicks (no force clamp).  Over 5 ps at 0.5 fs the
+    total energy of the relaxed 2,000-atom chain stays within 0.75 % of the
+    kinetic energy, and doubling dt raises the mean error between 2.5- and
+    6-fold.  Both bars come from the FP64 oracle run from this same state
+    (same minimised positions and Maxwell-Boltzmann draw): its maximum error
+    at 0.5 fs is 0.48 % of the kinetic energy (3.98 kcal/mol; the GPU 4.05)
+    and its mean-error ratio 1 fs / 0.5 fs is 3.5 (the GPU 3.3); from a state
+    relaxed by the oracle itself the ratio is 2.6.  The error is the
+    algorithm's -- the potential-shift cutoff keeps the energy continuous but
+    not the force, which adds a part that is not second order in dt -- and a
+    force error of the kernels, independent of dt, would break the ratio."""
     top, x, v = nve_start
     _, fine, ke = _nve_series(pm, top, x, v, 0.0005, 10, 0.5)
     _, coarse, _ = _nve_series(pm, top, x, v, 0.001, 10, 0.5)
     print(f"[nve] 0.5 fs: {np.round(fine, 2).tolist()}  1 fs: {np.round(coarse, 2).tolist()}  KE {ke:.0f}",
           flush=True)
-    assert np.abs(fine).max() < 0.01 * ke
+    assert np.abs(fine).max() < 0.0075 * ke
     ratio = np.abs(coarse).mean() / np.abs(fine).mean()
-    assert 2.5 < ratio < 10.0, ratio
+    assert 2.5 < ratio < 6.0, ratio
 
 
 def test_nve_drift_matches_oracle(pm, oracle, nve_start):
@@ -166,9 +169,12 @@ def test_coordinate_beyond_1000_A_is_a_divergence_fault(pm):
     import sys
     sys.path.insert(0, __file__.rsplit("/", 1)[0])
     from toy import make_topology
-    x = np.array([[999.5, 0.0, 0.0], [-20.0, 0.0, 0.0]], np.float32)
-    with pm.FusedEngine(make_topology(pm, x)) as eng:
-        eng.set_state(v=np.array([[200.0, 0.0, 0.0], [0.0, 0.0, 0.0]], np.float32))
+    # without friction, 50 A/ps (below the 20-thermal-speed clamp, 91 A/ps for
+    # 12 Da at 300 K) carries the atom 0.2 A per 4 fs step: past 1000 A in
+    # step 0 (the oracle puts it at 1000.1 A)
+    x = np.array([[999.9, 0.0, 0.0], [-20.0, 0.0, 0.0]], np.float32)
+    with pm.FusedEngine(make_topology(pm, x), pm.default_params(equilibration_steps=0, gamma_ps=0.0)) as eng:
+        eng.set_state(v=np.array([[50.0, 0.0, 0.0], [0.0, 0.0, 0.0]], np.float32))
         with pytest.raises(pm.PrismMdError) as ex:
             eng.step(3)
         assert ex.value.rc == pm.E_DIVERGED and "atom 0" in str(ex.value) and "step 0" in str(ex.value)
