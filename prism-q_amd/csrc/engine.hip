@@ -361,7 +361,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     P_.keys_b = dalloc<unsigned int>(A);
     P_.vals_a = dalloc<int>(A);
     P_.vals_b = dalloc<int>(A);
-    P_.com_partial = dalloc<double>((size_t)W * 3);
+    P_.com_partial = dalloc<double>((size_t)W * 12);
     P_.work_g = dalloc<double>((size_t)W * 64);
     P_.sweeps = dalloc<int>((size_t)W * 2);
     P_.counters = dalloc<EngineCounters>(1);

@@ -78,7 +78,7 @@ struct DevPtrs {
     // scratch
     unsigned int *keys_a, *keys_b;
     int *vals_a, *vals_b;
-    double *com_partial;            // [waves][3]
+    double *com_partial;            // [waves][4][3]: momentum sums of each 16-group quarter of a group wave
     double *work_g;                 // [waves * 64] per constraint group: cumulative non-Hamiltonian
                                     // work (bath, clamps), written only by the group's own lane
     int *sweeps;                    // [waves][2]: most SHAKE / RATTLE sweeps so far, per group wave

@@ -1,9 +1,11 @@
 // kernels_integrate.hip -- constrained Langevin OVRVO integrator.
 //
-// One lane per constraint group (a heavy atom with its <= 3 hydrogens, or a
-// lone atom).  Because every constraint couples atoms of one group only, all
-// per-atom and per-cluster phases between two force evaluations fuse into one
-// kernel; the reference launches each as a separate grid-wide phase
+// Work is laid out by constraint group (a heavy atom with its <= 3 hydrogens,
+// or a lone atom), a lane or a quad of lanes per group (the two layouts are
+// described above k_first_half1).  Because every constraint couples atoms of
+// one group only, all per-atom and per-cluster phases between two force
+// evaluations fuse into one kernel; the reference launches each as a
+// separate grid-wide phase
 // (fused_engine.rs:6223-6235, nhs_amber_fused.cu:3620-3850, 11128-11186):
 //   first half  (phases OU_HALF_N, RATTLE_AFTER_OU_N, KICK_N, RATTLE_PRE_DRIFT,
 //                DRIFT_SAVE_XN, SHAKE_POSITION), preceded by the COM removal of
@@ -12,13 +14,20 @@
 //                clamp, final RATTLE).
 // Kicks, Ornstein-Uhlenbeck updates and drifts are FP32 as in the reference;
 // SHAKE and RATTLE iterate in FP64 on the group's atoms.
+// Contraction into FMAs per source expression only (the C++ standard's rule),
+// here and in the shared device helpers: the two group layouts inline the
+// same phase functions into different kernels, and the default (fusing across
+// statements wherever the instruction selector finds a multiply feeding an
+// add) could round them differently.
+#pragma clang fp contract(on)
 #include "device_common.hpp"
 
 namespace prism {
 
 namespace {
 
-constexpr int IW = 4;  // waves per workgroup
+constexpr int IW = 4;  // waves per workgroup (the four-lane layout needs exactly 4: a workgroup per group wave)
+constexpr int QUAD_LAYOUT_MAX_WAVES = 512;
 constexpr int HS_GROUP = 128;  // workgroups per first-level counter of the step-end handshake
 constexpr unsigned HS_PUBLISHED = 1u << 16;  // count in the low 16 bits, publishers above
 
@@ -233,56 +242,68 @@ __device__ inline double ke_change(double m, double before2, double after2) {
     return 0.5 * m * (after2 - before2) * (1.0 / KCAL_TO_AMU_A2_PS2);
 }
 
-// langevin_thermostat with dt/2 (nhs_amber_fused.cu:1713-1744, :3656-3666),
-// booking each atom's kinetic-energy change in the work ledger.
-__device__ inline void ou_half(GroupState &g, const SysDesc &D, float T, float gamma, unsigned long long step,
-                               int purpose, const float *noise) {
-    const float c1 = expf(-gamma * (0.5f * D.dt));
-    FOR_ATOMS(k) {
-        float xi[3];
-        if (noise) {
-            for (int c = 0; c < 3; ++c) xi[c] = noise[(size_t)g.a[k] * 3 + c];
-        } else {
-            langevin_normals(D.seed_lo, D.stream, (uint32_t)(g.a[k] - D.atom_begin), step, purpose, xi);
-        }
-        // hardware reciprocal and square root (~1 ulp): the IEEE-rounded
-        // forms cost ~10 dependent instructions each on a latency-bound lane
-        const float c2 = __builtin_amdgcn_sqrtf((1.0f - c1 * c1) * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T *
-                                                __builtin_amdgcn_rcpf(g.m[k]));
-        const double before = speed2(g.v[k]);
-        for (int c = 0; c < 3; ++c) g.v[k][c] = c1 * g.v[k][c] + c2 * xi[c];
-        g.w += ke_change(g.m[k], before, speed2(g.v[k]));
+// ---- per-atom phases: one atom's state in, its update out.  Both group
+// layouts below call these, so they give bitwise the same trajectory.
+
+// langevin_thermostat with dt/2 (nhs_amber_fused.cu:1713-1744, :3656-3666)
+// of atom a; returns its kinetic-energy change for the work ledger.
+__device__ inline double ou_atom(float v[3], float m, int a, const SysDesc &D, float c1, float T,
+                                 unsigned long long step, int purpose, const float *noise) {
+    float xi[3];
+    if (noise) {
+        for (int c = 0; c < 3; ++c) xi[c] = noise[(size_t)a * 3 + c];
+    } else {
+        langevin_normals(D.seed_lo, D.stream, (uint32_t)(a - D.atom_begin), step, purpose, xi);
     }
+    // hardware reciprocal and square root (~1 ulp): the IEEE-rounded forms
+    // cost ~10 dependent instructions each on a latency-bound lane
+    const float c2 = __builtin_amdgcn_sqrtf((1.0f - c1 * c1) * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T *
+                                            __builtin_amdgcn_rcpf(m));
+    const double before = speed2(v);
+    for (int c = 0; c < 3; ++c) v[c] = c1 * v[c] + c2 * xi[c];
+    return ke_change(m, before, speed2(v));
 }
 
-// Half kick with the 1000 kcal/mol/A clamp of prism_asc_used_force
-// (prism_asc_decision.cuh:100-104) when gamma != 0, exact when gamma == 0
-// (nhs_amber_fused.cu:1750-1805, :3667-3703).
-// consume: zero the accumulators after reading them -- the step's force
-// evaluation adds into them (first half only).
-__device__ inline void half_kick(GroupState &g, const DevPtrs &P, float dt, float gamma, bool consume) {
-    FOR_ATOMS(k) {
-        float f[3];
-        for (int c = 0; c < 3; ++c) f[c] = from_fixed(g.f[k][c], FORCE_FIXED_BITS);
-        if (consume)
-            for (int c = 0; c < 3; ++c) P.f[(size_t)g.a[k] * 4 + c] = 0;
-        const float kick = 0.5f * dt * (float)KCAL_TO_AMU_A2_PS2 * __builtin_amdgcn_rcpf(g.m[k]);
-        if (gamma != 0.0f) {  // |f| > MAX_FORCE tested squared; the rescale is rare
-            const float mag2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
-            if (mag2 > MAX_FORCE * MAX_FORCE) {
-                // the clamp's work: used minus unclamped candidate velocity
-                // (apply_force_half_kick_with_intervention_work, :1750-1788)
-                float raw[3];
-                for (int c = 0; c < 3; ++c) raw[c] = g.v[k][c] + kick * f[c];
-                const float q = MAX_FORCE * __builtin_amdgcn_rsqf(mag2);
-                for (int c = 0; c < 3; ++c) f[c] *= q;
-                for (int c = 0; c < 3; ++c) g.v[k][c] += kick * f[c];
-                g.w += ke_change(g.m[k], speed2(raw), speed2(g.v[k]));
-                continue;
-            }
+// Half kick of one atom with the 1000 kcal/mol/A clamp of
+// prism_asc_used_force (prism_asc_decision.cuh:100-104) when gamma != 0,
+// exact when gamma == 0 (nhs_amber_fused.cu:1750-1805, :3667-3703).  Returns
+// true when the clamp acted, with its work -- used minus unclamped candidate
+// velocity (apply_force_half_kick_with_intervention_work, :1750-1788) -- in dw.
+__device__ inline bool kick_atom(float v[3], const long long fr[3], float m, float dt, float gamma, double &dw) {
+    float f[3];
+    for (int c = 0; c < 3; ++c) f[c] = from_fixed(fr[c], FORCE_FIXED_BITS);
+    const float kick = 0.5f * dt * (float)KCAL_TO_AMU_A2_PS2 * __builtin_amdgcn_rcpf(m);
+    if (gamma != 0.0f) {  // |f| > MAX_FORCE tested squared; the rescale is rare
+        const float mag2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        if (mag2 > MAX_FORCE * MAX_FORCE) {
+            float raw[3];
+            for (int c = 0; c < 3; ++c) raw[c] = v[c] + kick * f[c];
+            const float q = MAX_FORCE * __builtin_amdgcn_rsqf(mag2);
+            for (int c = 0; c < 3; ++c) f[c] *= q;
+            for (int c = 0; c < 3; ++c) v[c] += kick * f[c];
+            dw = ke_change(m, speed2(raw), speed2(v));
+            return true;
         }
-        for (int c = 0; c < 3; ++c) g.v[k][c] += kick * f[c];
     }
+    for (int c = 0; c < 3; ++c) v[c] += kick * f[c];
+    return false;
+}
+
+// nhs_velocity_safety_clamp (nhs_amber_fused.cu:11148-11186) of one atom:
+// |v| > max(20 sqrt(418.4 kB T / max(m, 1)), 2), tested squared.  Returns
+// true when it acted, with the kinetic-energy change in dw.
+__device__ inline bool vclamp_atom(float v[3], float m, float T, double &dw) {
+    const float vmax2 = fmaxf(400.0f * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T *
+                                  __builtin_amdgcn_rcpf(fmaxf(m, 1.0f)), 4.0f);
+    const float sp2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (sp2 > vmax2) {
+        const double before = speed2(v);
+        const float sc = __builtin_amdgcn_sqrtf(vmax2) * __builtin_amdgcn_rsqf(sp2);
+        for (int c = 0; c < 3; ++c) v[c] *= sc;
+        dw = ke_change(m, before, speed2(v));
+        return true;
+    }
+    return false;
 }
 
 // Kinetic energy change of a RATTLE projection booked to the work ledger
@@ -358,14 +379,119 @@ __device__ inline void note_sweeps(SysState &S, int sweeps, int kind, int atom, 
     else most = max(most, sweeps);
 }
 // Sweep statistics: the wave's maximum raises the running maximum `prev`
-// (read at kernel start) of this wave's own slot; no atomics, called in
-// converged control flow.
+// (read at kernel start) of its group wave's slot, which the four kernel
+// waves of the four-lane layout share -- hence the atomic, issued only when
+// it raises.  Called in converged control flow.
 __device__ inline void publish_sweeps(int *slot, int prev, int most) {
     for (int o = 32; o > 0; o >>= 1) most = max(most, __shfl_xor(most, o));
-    if ((threadIdx.x & 63) == 0 && most > prev) *slot = most;
+    if ((threadIdx.x & 63) == 0 && most > prev) atomicMax(slot, most);
 }
 
-__global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *noise) {
+// A coordinate beyond 1000 A means the system has diverged (heartbeat_check,
+// housekeeping.cu:27-45); a non-finite one is a fault of its own.
+__device__ inline void check_atom(SysState &S, const float x[3], const float v[3], int a, unsigned long long step) {
+    if (!(finite3(x) && finite3(v))) record_fault(S, FAULT_NONFINITE, a, step);
+    else if (fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fabsf(x[2])) > MAX_COORD) record_fault(S, FAULT_DIVERGED, a, step);
+}
+
+// ---- group layouts
+//
+// Layout 1 (a lane per group): a lane holds a whole group, up to 4 atoms, and
+// runs every phase for its atoms in turn.
+// Layout 4 (a quad of lanes per group): lane k of the quad holds atom k.  The
+// per-atom phases (the Ornstein-Uhlenbeck update with its Philox draws, the
+// kick, the clamps, the loads and stores) run once per lane instead of four
+// times in a row; every lane of the quad also holds a copy of the group's
+// positions and velocities, exchanged with quad broadcasts (DPP), on which it
+// runs the cluster's RATTLE/SHAKE -- the same FP64 operations on the same
+// inputs as layout 1.  The work ledger gathers the lanes' contributions in
+// atom order, and the centre-of-mass partials are summed in one tree in both
+// layouts, so the two give bitwise the same positions, velocities and work.
+// A small arena -- fewer group waves than the GPU has SIMDs -- leaves most of
+// the chip idle and is bound by the length of one lane's dependent chain,
+// which layout 4 shortens; a large arena is bound by throughput, where
+// layout 1 does no work four times over.
+
+// The value `v` of lane J of this lane's quad (DPP quad_perm [J,J,J,J]);
+// every lane of the quad must be active.
+template <int J>
+__device__ inline int quad_lane(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, J * 0x55, 0xF, 0xF, false);
+}
+template <int J>
+__device__ inline float quad_lane(float v) {
+    return __int_as_float(quad_lane<J>(__float_as_int(v)));
+}
+template <int J>
+__device__ inline double quad_lane(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)quad_lane<J>((int)(unsigned)(b & 0xffffffffll));
+    const unsigned hi = (unsigned)quad_lane<J>((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The quad's positions and velocities into every lane's group copy.
+__device__ inline void quad_gather_xv(GroupState &g, const float x[3], const float v[3]) {
+#define PRISM_QG(J)                                \
+    for (int c = 0; c < 3; ++c) {                  \
+        g.x[J][c] = quad_lane<J>(x[c]);            \
+        g.v[J][c] = quad_lane<J>(v[c]);            \
+    }
+    PRISM_QG(0) PRISM_QG(1) PRISM_QG(2) PRISM_QG(3)
+#undef PRISM_QG
+}
+__device__ inline void quad_gather_v(GroupState &g, const float v[3]) {
+#define PRISM_QG(J) \
+    for (int c = 0; c < 3; ++c) g.v[J][c] = quad_lane<J>(v[c]);
+    PRISM_QG(0) PRISM_QG(1) PRISM_QG(2) PRISM_QG(3)
+#undef PRISM_QG
+}
+// Atom k's row of a group array, selected among values: handed the array
+// itself, the compiler folds the selects into one dynamically indexed load,
+// which moves the whole group state to scratch memory.
+__device__ inline float pick4(int k, float a, float b, float c, float d) {
+    return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
+}
+#define PRISM_OWN_ROW(src, k, out) \
+    for (int c_ = 0; c_ < 3; ++c_) (out)[c_] = pick4((k), (src)[0][c_], (src)[1][c_], (src)[2][c_], (src)[3][c_])
+// w += the contributions `c` of the quad's lanes whose `has` is set, in lane
+// (atom) order: the order in which layout 1 adds its atoms' contributions.
+// Every lane of the quad ends with the same w.
+__device__ inline void quad_book(double &w, double c, bool has) {
+    const int h = has ? 1 : 0;
+#define PRISM_QB(J)                        \
+    {                                      \
+        const double cj = quad_lane<J>(c); \
+        if (quad_lane<J>(h)) w += cj;      \
+    }
+    PRISM_QB(0) PRISM_QB(1) PRISM_QB(2) PRISM_QB(3)
+#undef PRISM_QB
+}
+
+// Mass-weighted velocity sum of a group over its atoms in order.
+__device__ inline void group_momentum(const GroupState &g, double mv[3]) {
+    FOR_ATOMS(k) {
+        for (int c = 0; c < 3; ++c) mv[c] += (double)g.m[k] * g.v[k][c];
+    }
+}
+// Sum of the groups of a 16-group sub-block of a group wave: a butterfly over
+// the groups' lanes at distances `first`, 2 first, 4 first, 8 first (groups 1,
+// 2, 4, 8 apart in both layouts), so both layouts form the same tree.
+__device__ inline double sum16(double v, int first) {
+    for (int o = first; o < 16 * first; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ inline void freeze_atom(const DevPtrs &P, int a) {
+    if (a >= 0)
+        for (int c = 0; c < 3; ++c) P.f[(size_t)a * 4 + c] = 0;
+}
+
+// ---- first half: COM removal, OU_HALF_N, RATTLE_AFTER_OU_N (booked),
+// KICK_N, RATTLE_PRE_DRIFT, DRIFT_SAVE_XN, SHAKE_POSITION, then the
+// divergence checks, the stores and the displacement trigger of the pair list.
+
+__global__ __launch_bounds__(64 * IW) void k_first_half1(DevPtrs P, const float *noise) {
     const int lane = threadIdx.x & 63;
     const int gw = blockIdx.x * IW + wave_id();
     if (gw >= P.n_waves_arena) return;
@@ -376,9 +502,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
     SysState &S = P.sst[s];
     if (S.halted || parked(D, S)) {  // frozen, but its forces are still re-evaluated into zeroed accumulators
         const Group G = P.groups[gw * 64 + lane];
-        for (int k = 0; k < 4; ++k)
-            if (G.atom[k] >= 0)
-                for (int c = 0; c < 3; ++c) P.f[(size_t)G.atom[k] * 4 + c] = 0;
+        for (int k = 0; k < 4; ++k) freeze_atom(P, G.atom[k]);
         return;
     }
     const unsigned long long step = S.step;
@@ -397,9 +521,16 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
                 for (int c = 0; c < 3; ++c) g.v[k][c] -= (float)S.com_v[c];
             }
         }
-        if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 1, noise);
+        if (gamma != 0.0f) {
+            const float c1 = expf(-gamma * (0.5f * D.dt));
+            FOR_ATOMS(k) g.w += ou_atom(g.v[k], g.m[k], g.a[k], D, c1, T, step, 1, noise);
+        }
         note_sweeps(S, rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_1, g.a[0], step, most_ra);
-        half_kick(g, P, D.dt, gamma, true);
+        FOR_ATOMS(k) {
+            freeze_atom(P, g.a[k]);  // the step's force evaluation adds into the zeroed accumulator
+            double dw;
+            if (kick_atom(g.v[k], g.f[k], g.m[k], D.dt, gamma, dw)) g.w += dw;
+        }
         note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_2, g.a[0], step, most_ra);
         float xs[4][3];
         FOR_ATOMS(k) {
@@ -411,11 +542,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
         note_sweeps(S, shake(g, xs, gp, D.dt, D.shake_tol, D.constraint_max_iter), FAULT_SHAKE, g.a[0], step, most_sh);
         FOR_ATOMS(k) {
             const int a = g.a[k];
-            if (!(finite3(g.x[k]) && finite3(g.v[k]))) record_fault(S, FAULT_NONFINITE, a, step);
-            // heartbeat_check (housekeeping.cu:27-45): a coordinate beyond
-            // 1000 A means the system has diverged
-            else if (fmaxf(fmaxf(fabsf(g.x[k][0]), fabsf(g.x[k][1])), fabsf(g.x[k][2])) > MAX_COORD)
-                record_fault(S, FAULT_DIVERGED, a, step);
+            check_atom(S, g.x[k], g.v[k], a, step);
             P.x[a] = make_float4(g.x[k][0], g.x[k][1], g.x[k][2], 0.0f);
             P.v[a] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
             float *q = (float *)&P.xq[tail.slot[k]];
@@ -436,76 +563,120 @@ __global__ __launch_bounds__(64 * IW) void k_first_half(DevPtrs P, const float *
     }
 }
 
-__global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float *noise) {
-    const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * IW + wave_id();
-    __shared__ int last;
-    bool publish = false;  // this thread wrote something the last workgroup reads
-    const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
-    if (s >= 0 && P.sys[s].active && !P.sst[s].halted && !parked(P.sys[s], P.sst[s])) {
-        const SysDesc &D = P.sys[s];
-        SysState &S = P.sst[s];
-        const unsigned long long step = S.step;
-        float T, gamma;
-        schedule(D, step + 1, T, gamma);
-        const bool next_com = D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0;
-        const int prev_ra = P.sweeps[2 * gw + 1];
-        GroupState g;
-        GroupParam gp;
-        GroupTail unused;
-        double mv[3] = {0.0, 0.0, 0.0};
-        int most_ra = 0;
-        if (load_group<false>(P, gw * 64 + lane, g, gp, unused)) {
-            half_kick(g, P, D.dt, gamma, false);
-            int sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
-            publish |= sw < 0;
-            note_sweeps(S, sw, FAULT_RATTLE_3, g.a[0], step, most_ra);
-            if (gamma != 0.0f) ou_half(g, D, T, gamma, step, 2, noise);
-            FOR_ATOMS(k) {  // nhs_velocity_safety_clamp (nhs_amber_fused.cu:11148-11186)
-                // |v| > max(20 sqrt(418.4 kB T / max(m, 1)), 2) tested squared
-                const float vmax2 = fmaxf(400.0f * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T *
-                                              __builtin_amdgcn_rcpf(fmaxf(g.m[k], 1.0f)), 4.0f);
-                const float sp2 = g.v[k][0] * g.v[k][0] + g.v[k][1] * g.v[k][1] + g.v[k][2] * g.v[k][2];
-                if (sp2 > vmax2) {
-                    const double before = speed2(g.v[k]);
-                    const float sc = __builtin_amdgcn_sqrtf(vmax2) * __builtin_amdgcn_rsqf(sp2);
-                    for (int c = 0; c < 3; ++c) g.v[k][c] *= sc;
-                    g.w += ke_change(g.m[k], before, speed2(g.v[k]));
-                }
-            }
-            sw = rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter);
-            publish |= sw < 0;
-            note_sweeps(S, sw, FAULT_RATTLE_4, g.a[0], step, most_ra);
-            FOR_ATOMS(k) {
-                if (!finite3(g.v[k])) {
-                    record_fault(S, FAULT_NONFINITE, g.a[k], step);
-                    publish = true;
-                }
-                P.v[g.a[k]] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
-                for (int c = 0; c < 3; ++c) mv[c] += (double)g.m[k] * g.v[k][c];
-            }
-            P.work_g[gw * 64 + lane] = g.w;
+// Layout 4: kernel wave kw covers groups 16 (kw & 3) .. 16 (kw & 3) + 15 of
+// group wave kw >> 2; a workgroup is one group wave.
+__global__ __launch_bounds__(64 * IW) void k_first_half4(DevPtrs P, const float *noise) {
+    const int lane = threadIdx.x & 63, k = lane & 3;
+    const int kw = blockIdx.x * IW + wave_id();
+    const int gw = kw >> 2;
+    if (gw >= P.n_waves_arena) return;
+    const int gidx = gw * 64 + (kw & 3) * 16 + (lane >> 2);
+    // the group's records load before the system's: they depend on nothing
+    // that the system's records decide
+    const int a = P.groups[gidx].atom[k];  // this lane's atom, -1 if the group has none here
+    const GroupParam gp = P.gparams[gidx];
+    const int s = P.wave_sys[gw];
+    if (s < 0) return;
+    const SysDesc &D = P.sys[s];
+    if (!D.active) return;
+    SysState &S = P.sst[s];
+    if (S.halted || parked(D, S)) {
+        freeze_atom(P, a);
+        return;
+    }
+    const size_t ac = (size_t)max(a, 0);  // an absent atom reads arena atom 0, never used
+    const float m = P.mass[ac];
+    const float4 xa = P.x[ac], va = P.v[ac];
+    long long fr[3];
+    for (int c = 0; c < 3; ++c) fr[c] = P.f[ac * 4 + c];
+    const int slot = P.slot_of[ac];
+    const float4 xb = P.x_build[ac];
+    const unsigned long long step = S.step;
+    float T, gamma;
+    schedule(D, step + 1, T, gamma);
+    const bool com = D.com_every > 0 && step > 0 && step % (unsigned long long)D.com_every == 0;
+    const int prev_sh = P.sweeps[2 * gw], prev_ra = P.sweeps[2 * gw + 1];
+    GroupState g;
+    g.w = P.work_g[gidx];
+    bool moved = false;
+    int most_sh = 0, most_ra = 0;
+    if (quad_lane<0>(a) >= 0) {  // quad-uniform: not a padding group
+        g.n = 1 + gp.n_h;
+        const bool mine = k < g.n;
+        float x[3] = {xa.x, xa.y, xa.z}, v[3] = {va.x, va.y, va.z};
+        if (com)
+            for (int c = 0; c < 3; ++c) v[c] -= (float)S.com_v[c];
+        if (gamma != 0.0f) {
+            const float c1 = expf(-gamma * (0.5f * D.dt));
+            double dw = 0.0;
+            if (mine) dw = ou_atom(v, m, a, D, c1, T, step, 1, noise);
+            quad_book(g.w, dw, mine);
         }
-        publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
-        if (next_com) {
-            for (int c = 0; c < 3; ++c) mv[c] = wave_sum(mv[c]);
-            if (lane == 0) {
-                for (int c = 0; c < 3; ++c) P.com_partial[(size_t)gw * 3 + c] = mv[c];
-                publish = true;
+        quad_gather_xv(g, x, v);
+        int r = rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter);
+        if (k == 0) note_sweeps(S, r, FAULT_RATTLE_1, a, step, most_ra);
+        PRISM_OWN_ROW(g.v, k, v);
+        {
+            double dw = 0.0;
+            bool hit = false;
+            if (mine) {
+                freeze_atom(P, a);
+                hit = kick_atom(v, fr, m, D.dt, gamma, dw);
             }
+            quad_book(g.w, dw, hit);
+        }
+        quad_gather_v(g, v);
+        r = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+        if (k == 0) note_sweeps(S, r, FAULT_RATTLE_2, a, step, most_ra);
+        float xs[4][3];
+        FOR_ATOMS(j) {
+            for (int c = 0; c < 3; ++c) {
+                xs[j][c] = g.x[j][c];
+                g.x[j][c] += D.dt * g.v[j][c];
+            }
+        }
+        r = shake(g, xs, gp, D.dt, D.shake_tol, D.constraint_max_iter);
+        if (k == 0) note_sweeps(S, r, FAULT_SHAKE, a, step, most_sh);
+        PRISM_OWN_ROW(g.x, k, x);
+        PRISM_OWN_ROW(g.v, k, v);
+        if (mine) {
+            check_atom(S, x, v, a, step);
+            P.x[a] = make_float4(x[0], x[1], x[2], 0.0f);
+            P.v[a] = make_float4(v[0], v[1], v[2], 0.0f);
+            float *q = (float *)&P.xq[slot];
+            q[0] = x[0]; q[1] = x[1]; q[2] = x[2];
+            const float dx = x[0] - xb.x, dy = x[1] - xb.y, dz = x[2] - xb.z;
+            moved = dx * dx + dy * dy + dz * dz > D.half_skin2;
+        }
+        if (k == 0) P.work_g[gidx] = g.w;
+    }
+    publish_sweeps(P.sweeps + 2 * gw, prev_sh, most_sh);
+    publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
+    const bool due = (long long)step - S.rebuild_step >= (long long)D.rebuild_every;
+    if (__ballot(moved) != 0ull || due) {
+        if (lane == 0) {
+            S.rebuild_flag = 1;
+            P.counters->any_rebuild = 1;
         }
     }
-    // Last-block handshake (no waiting): the workgroup that retires last sums
-    // the COM partials in wave order and advances every system's step counter.
-    // A workgroup releases its writes (agent-scope fence) only when the last
-    // one reads any of them -- COM partials, a latched fault: the fence writes
-    // back this XCD's L2 and costs microseconds per workgroup, so on the other
-    // steps the counter increment goes out unfenced.  The count has two
-    // levels, HS_GROUP workgroups per group counter and one increment per
-    // group on the global one: atomics on one address serialise (~12 ns
-    // each), which a flat count over the thousands of workgroups of a large
-    // arena would pay on every step.  A workgroup that published adds
-    // HS_PUBLISHED, so the last of its group knows to pass the writes on.
+}
+
+// ---- second half: KICK_NP1, RATTLE_BEFORE_OU_NP1, OU_HALF_NP1, the velocity
+// safety clamp and the final RATTLE (booked); then the step-end handshake.
+
+// Last-block handshake (no waiting): the workgroup that retires last sums the
+// COM partials in a fixed order and advances every system's step counter.  A
+// workgroup releases its writes (agent-scope fence) only when the last one
+// reads any of them -- COM partials, a latched fault: the fence writes back
+// this XCD's L2 and costs microseconds per workgroup, so on the other steps
+// the counter increment goes out unfenced.  The count has two levels,
+// HS_GROUP workgroups per group counter and one increment per group on the
+// global one: atomics on one address serialise (~12 ns each), which a flat
+// count over the thousands of workgroups of a large arena would pay on every
+// step.  A workgroup that published adds HS_PUBLISHED, so the last of its
+// group knows to pass the writes on.
+__device__ inline void step_end(const DevPtrs &P, bool publish) {
+    __shared__ int last;
     const int pub = __syncthreads_or(publish);
     if (pub) __threadfence();
     __syncthreads();
@@ -538,9 +709,13 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         if (S.halted || parked(D, S)) continue;
         const unsigned long long step = S.step;
         if (D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0) {
+            // each group wave's four 16-group sums, (s0 + s1) + (s2 + s3),
+            // then the group waves in order
             double acc[3] = {0.0, 0.0, 0.0};
-            for (int w = D.wave_begin; w < D.wave_begin + D.n_waves; ++w)
-                for (int c = 0; c < 3; ++c) acc[c] += P.com_partial[(size_t)w * 3 + c];
+            for (int w = D.wave_begin; w < D.wave_begin + D.n_waves; ++w) {
+                const double *p = P.com_partial + (size_t)w * 12;
+                for (int c = 0; c < 3; ++c) acc[c] += (p[c] + p[3 + c]) + (p[6 + c] + p[9 + c]);
+            }
             for (int c = 0; c < 3; ++c) S.com_v[c] = acc[c] * D.inv_total_mass;
         }
         if (S.rebuild_flag) {  // this step's rebuild is done: consume its flags
@@ -564,6 +739,143 @@ __global__ __launch_bounds__(64 * IW) void k_second_half(DevPtrs P, const float 
         P.step_ts[k % TS_RING] = (unsigned long long)wall_clock64();
         P.counters->step_ends = k + 1;
     }
+}
+
+__global__ __launch_bounds__(64 * IW) void k_second_half1(DevPtrs P, const float *noise) {
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * IW + wave_id();
+    bool publish = false;  // this thread wrote something the last workgroup reads
+    const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
+    if (s >= 0 && P.sys[s].active && !P.sst[s].halted && !parked(P.sys[s], P.sst[s])) {
+        const SysDesc &D = P.sys[s];
+        SysState &S = P.sst[s];
+        const unsigned long long step = S.step;
+        float T, gamma;
+        schedule(D, step + 1, T, gamma);
+        const bool next_com = D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0;
+        const int prev_ra = P.sweeps[2 * gw + 1];
+        GroupState g;
+        GroupParam gp;
+        GroupTail unused;
+        double mv[3] = {0.0, 0.0, 0.0};
+        int most_ra = 0;
+        if (load_group<false>(P, gw * 64 + lane, g, gp, unused)) {
+            FOR_ATOMS(k) {
+                double dw;
+                if (kick_atom(g.v[k], g.f[k], g.m[k], D.dt, gamma, dw)) g.w += dw;
+            }
+            int sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            publish |= sw < 0;
+            note_sweeps(S, sw, FAULT_RATTLE_3, g.a[0], step, most_ra);
+            if (gamma != 0.0f) {
+                const float c1 = expf(-gamma * (0.5f * D.dt));
+                FOR_ATOMS(k) g.w += ou_atom(g.v[k], g.m[k], g.a[k], D, c1, T, step, 2, noise);
+            }
+            FOR_ATOMS(k) {
+                double dw;
+                if (vclamp_atom(g.v[k], g.m[k], T, dw)) g.w += dw;
+            }
+            sw = rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter);
+            publish |= sw < 0;
+            note_sweeps(S, sw, FAULT_RATTLE_4, g.a[0], step, most_ra);
+            FOR_ATOMS(k) {
+                if (!finite3(g.v[k])) {
+                    record_fault(S, FAULT_NONFINITE, g.a[k], step);
+                    publish = true;
+                }
+                P.v[g.a[k]] = make_float4(g.v[k][0], g.v[k][1], g.v[k][2], 0.0f);
+            }
+            group_momentum(g, mv);
+            P.work_g[gw * 64 + lane] = g.w;
+        }
+        publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
+        if (next_com) {
+            for (int c = 0; c < 3; ++c) mv[c] = sum16(mv[c], 1);
+            if ((lane & 15) == 0) {
+                for (int c = 0; c < 3; ++c) P.com_partial[((size_t)gw * 4 + (lane >> 4)) * 3 + c] = mv[c];
+                publish = true;
+            }
+        }
+    }
+    step_end(P, publish);
+}
+
+__global__ __launch_bounds__(64 * IW) void k_second_half4(DevPtrs P, const float *noise) {
+    const int lane = threadIdx.x & 63, k = lane & 3;
+    const int kw = blockIdx.x * IW + wave_id();
+    const int gw = kw >> 2;
+    bool publish = false;
+    const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
+    if (s >= 0 && P.sys[s].active && !P.sst[s].halted && !parked(P.sys[s], P.sst[s])) {
+        const SysDesc &D = P.sys[s];
+        SysState &S = P.sst[s];
+        const int gidx = gw * 64 + (kw & 3) * 16 + (lane >> 2);
+        const int a = P.groups[gidx].atom[k];
+        const GroupParam gp = P.gparams[gidx];
+        const size_t ac = (size_t)max(a, 0);
+        const float m = P.mass[ac];
+        const float4 xa = P.x[ac], va = P.v[ac];
+        long long fr[3];
+        for (int c = 0; c < 3; ++c) fr[c] = P.f[ac * 4 + c];
+        const unsigned long long step = S.step;
+        float T, gamma;
+        schedule(D, step + 1, T, gamma);
+        const bool next_com = D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0;
+        const int prev_ra = P.sweeps[2 * gw + 1];
+        GroupState g;
+        g.w = P.work_g[gidx];
+        double mv[3] = {0.0, 0.0, 0.0};
+        int most_ra = 0;
+        if (quad_lane<0>(a) >= 0) {
+            g.n = 1 + gp.n_h;
+            const bool mine = k < g.n;
+            float x[3] = {xa.x, xa.y, xa.z}, v[3] = {va.x, va.y, va.z};
+            {
+                double dw = 0.0;
+                bool hit = false;
+                if (mine) hit = kick_atom(v, fr, m, D.dt, gamma, dw);
+                quad_book(g.w, dw, hit);
+            }
+            quad_gather_xv(g, x, v);
+            int sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            publish |= sw < 0;
+            if (k == 0) note_sweeps(S, sw, FAULT_RATTLE_3, a, step, most_ra);
+            PRISM_OWN_ROW(g.v, k, v);
+            double d_ou = 0.0, d_vc = 0.0;
+            bool h_vc = false;
+            if (gamma != 0.0f) {
+                const float c1 = expf(-gamma * (0.5f * D.dt));
+                if (mine) d_ou = ou_atom(v, m, a, D, c1, T, step, 2, noise);
+                quad_book(g.w, d_ou, mine);
+            }
+            if (mine) h_vc = vclamp_atom(v, m, T, d_vc);
+            quad_book(g.w, d_vc, h_vc);
+            quad_gather_v(g, v);
+            sw = rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter);
+            publish |= sw < 0;
+            if (k == 0) note_sweeps(S, sw, FAULT_RATTLE_4, a, step, most_ra);
+            PRISM_OWN_ROW(g.v, k, v);
+            if (mine) {
+                if (!finite3(v)) {
+                    record_fault(S, FAULT_NONFINITE, a, step);
+                    publish = true;
+                }
+                P.v[a] = make_float4(v[0], v[1], v[2], 0.0f);
+            }
+            g.m[0] = quad_lane<0>(m); g.m[1] = quad_lane<1>(m); g.m[2] = quad_lane<2>(m); g.m[3] = quad_lane<3>(m);
+            group_momentum(g, mv);
+            if (k == 0) P.work_g[gidx] = g.w;
+        }
+        publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
+        if (next_com) {
+            for (int c = 0; c < 3; ++c) mv[c] = sum16(mv[c], 4);
+            if (lane == 0) {
+                for (int c = 0; c < 3; ++c) P.com_partial[((size_t)gw * 4 + (kw & 3)) * 3 + c] = mv[c];
+                publish = true;
+            }
+        }
+    }
+    step_end(P, publish);
 }
 
 // Per-system reductions in FP64 (one workgroup per system, fixed order):
@@ -608,14 +920,23 @@ __global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs P, double *out /*[S]
 
 }  // namespace
 
+// The four-lane layout when the arena's group waves are few enough that the
+// integrator is bound by one lane's chain (each group wave becomes four
+// kernel waves: at this bound, up to 2,048 of them on the GPU's 1,024 SIMDs).
+static bool quad_layout(const DevPtrs &P) { return P.n_waves_arena <= QUAD_LAYOUT_MAX_WAVES; }
+
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t st) {
-    const int grid = (P.n_waves_arena + IW - 1) / IW;
-    hipLaunchKernelGGL(k_first_half, dim3(grid), dim3(64 * IW), 0, st, P, noise);
+    if (quad_layout(P))
+        hipLaunchKernelGGL(k_first_half4, dim3(P.n_waves_arena), dim3(64 * IW), 0, st, P, noise);
+    else
+        hipLaunchKernelGGL(k_first_half1, dim3((P.n_waves_arena + IW - 1) / IW), dim3(64 * IW), 0, st, P, noise);
 }
 
 void launch_second_half(const DevPtrs &P, const float *noise, hipStream_t st) {
-    const int grid = (P.n_waves_arena + IW - 1) / IW;
-    hipLaunchKernelGGL(k_second_half, dim3(grid), dim3(64 * IW), 0, st, P, noise);
+    if (quad_layout(P))
+        hipLaunchKernelGGL(k_second_half4, dim3(P.n_waves_arena), dim3(64 * IW), 0, st, P, noise);
+    else
+        hipLaunchKernelGGL(k_second_half1, dim3((P.n_waves_arena + IW - 1) / IW), dim3(64 * IW), 0, st, P, noise);
 }
 
 void launch_kinetic(const DevPtrs &P, double *out, int com_sys, hipStream_t st) {
