@@ -74,6 +74,25 @@ __device__ inline void tile_pairs(const float4 xi, const float2 li, int n_iter, 
 
 constexpr int N_XCD = 8;
 
+// Value of lane (l - R) mod 16 of the same 16-lane row: DPP row_ror, no LDS
+// traffic.  Four rotations by 1, 2, 4, 8 leave every lane of a row with the
+// row's minimum or maximum.
+template <int R> __device__ inline float row_ror(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + R, 0xf, 0xf, false));
+}
+__device__ inline float row_min(float v) {
+    v = fminf(v, row_ror<1>(v));
+    v = fminf(v, row_ror<2>(v));
+    v = fminf(v, row_ror<4>(v));
+    return fminf(v, row_ror<8>(v));
+}
+__device__ inline float row_max(float v) {
+    v = fmaxf(v, row_ror<1>(v));
+    v = fmaxf(v, row_ror<2>(v));
+    v = fmaxf(v, row_ror<4>(v));
+    return fmaxf(v, row_ror<8>(v));
+}
+
 __device__ inline long long wave_xor_ll(long long v, int m) {
     const int lo = __shfl_xor((int)(v & 0xffffffffll), m), hi = __shfl_xor((int)(v >> 32), m);
     return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
@@ -146,6 +165,8 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
             const int ti = min(t0 + (lane & (CHUNK_MAX - 1)), MAX_ROW - 1);
             const int my_jb = P.row_j[(size_t)b * MAX_ROW + ti];
             const int my_ms = P.row_m[(size_t)b * MAX_ROW + ti];
+            // and the build-time box of each tile's j-block
+            const float4 tb_c = P.bb_c[my_jb], tb_h = P.bb_h[my_jb];
             const int s = P.block_sys[b];
             const SysDesc &D = P.sys[s];
             const int nrow = P.row_n[b];
@@ -169,18 +190,30 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 hi[0] = hi[1] = hi[2] = -3.0e38f;
             }
             // the four 16-lane rows hold the same 16 atoms: reduce within a row
-            for (int k = 0; k < 3; ++k)
-                for (int o = 1; o < SUB; o <<= 1) {
-                    lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
-                    hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
-                }
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = row_min(lo[k]);
+                hi[k] = row_max(hi[k]);
+            }
             const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
             const float hx = 0.5f * (hi[0] - lo[0]), hy = 0.5f * (hi[1] - lo[1]), hz = 0.5f * (hi[2] - lo[2]);
 
-            // the unit's tile indices and mask slots, one per lane, then a software
-            // pipeline: the next tile's j-atoms load while this tile computes
+            // The unit's live tiles: between rebuilds no atom is more than
+            // skin/2 from its build position (the rebuild trigger), so a tile
+            // whose j-block build box, widened by skin/2, lies beyond the
+            // cutoff of the quarter's current box holds no pair and is skipped
+            // (about a third of a row's tiles are, for a block quarter).
             const int n_t = t1 - t0;
-            int jb = __shfl(my_jb, 0);
+            const float drift = sqrtf(D.half_skin2) + 0.01f;
+            const float tx = fmaxf(fabsf(cx - tb_c.x) - hx - tb_h.x - drift, 0.0f);
+            const float ty = fmaxf(fabsf(cy - tb_c.y) - hy - tb_h.y - drift, 0.0f);
+            const float tz = fmaxf(fabsf(cz - tb_c.z) - hz - tb_h.z - drift, 0.0f);
+            unsigned long long tiles = __ballot(lane < n_t && tx * tx + ty * ty + tz * tz <= reach2);
+            const bool any_tile = tiles != 0ull;
+            // then a software pipeline over the live tiles: the next tile's
+            // j-atoms load while this tile computes
+            int k = any_tile ? __ffsll((long long)tiles) - 1 : 0;
+            tiles &= tiles - 1ull;
+            int jb = __builtin_amdgcn_readlane(my_jb, k);
             float4 xj = P.xq[jb * BLOCK + lane];
             float2 lj = P.lj[jb * BLOCK + lane];
             float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
@@ -189,12 +222,16 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
             // appends wrap at most once (queued < RING/2 before an append of
             // at most 64).  Wave-uniform.
             int head = 0, queued = 0;
-            for (int k = 0; k < n_t; ++k) {
-                const int ms = __shfl(my_ms, k);
+            for (bool more = any_tile; more;) {
+                const int ms = __builtin_amdgcn_readlane(my_ms, k);
                 float4 xn = xj;
                 float2 ln = lj;
-                if (k + 1 < n_t) {
-                    const int jn = __shfl(my_jb, k + 1);
+                int kn = k;
+                more = tiles != 0ull;
+                if (more) {
+                    kn = __ffsll((long long)tiles) - 1;
+                    tiles &= tiles - 1ull;
+                    const int jn = __builtin_amdgcn_readlane(my_jb, kn);
                     xn = P.xq[jn * BLOCK + lane];
                     ln = P.lj[jn * BLOCK + lane];
                 }
@@ -207,7 +244,7 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 if (ms >= 0 && n_act > 0) {
                     // a tile with skipped pairs: its survivors now, with their mask word
                     if (active) {
-                        const int pos = __popcll(m & lt);
+                        const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                         jpos[pos] = xj;
                         jpar[pos] = make_float4(lj.x, lj.y, __int_as_float(lane), 0.0f);
                     }
@@ -223,7 +260,7 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                     wave_lds_fence();
                 } else if (n_act > 0) {
                     if (active) {
-                        const int pos = (head + queued + __popcll(m & lt)) & (RING - 1);
+                        const int pos = (head + queued + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))) & (RING - 1);
                         rpos[pos] = xj;
                         rpar[pos] = make_float4(lj.x, lj.y, 0.0f, 0.0f);
                     }
@@ -239,6 +276,7 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 }
                 xj = xn;
                 lj = ln;
+                k = kn;
             }
             if (queued > 0) {  // the unit's last, partial batch, padded to whole iterations
                 const int n_pad = (queued + 3) & ~3;
