@@ -371,10 +371,15 @@ def serving_leg(pm, seconds, rate, budget_gib, graph_steps, seed=0x50524953 + 5)
     pending = collections.deque()
     t0 = time.perf_counter()
     submitted = 0
+    next_report = 60.0
     while True:
         now = time.perf_counter() - t0
         if now >= seconds:
             break
+        if now >= next_report:  # a progress line per minute on long runs
+            log(f"serving: {now:.0f} s, {submitted} submitted, {len(sch.done)} done, {len(sch.failed)} faulted, "
+                f"{len(sch.running)} running")
+            next_report += 60.0
         while True:
             try:
                 pending.append(ready.get_nowait())
