@@ -163,7 +163,7 @@ typedef struct PrismMdParamsV1 {
     float    sort_cell_A;         /* edge of the Hilbert sort cells that order atoms into
                                    * 64-atom blocks (2.0; 0 selects 2.0)            */
     int32_t  chunk_tiles;         /* row tiles per nonbonded work unit, 8..32; 0 picks
-                                   * 10 below 100,000 atoms, 24 above               */
+                                   * 8 below 100,000 atoms, 24 above                */
     int32_t  reserved[5];
 } PrismMdParamsV1;
 

@@ -365,6 +365,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     P_.work_g = dalloc<double>((size_t)W * 64);
     P_.sweeps = dalloc<int>((size_t)W * 2);
     P_.counters = dalloc<EngineCounters>(1);
+    P_.nb_claim = dalloc<unsigned>((size_t)NB_CLAIM_COUNTERS * NB_CLAIM_STRIDE);  // zeroed
     P_.step_ts = dalloc<unsigned long long>(TS_RING);
     P_.hs_group = dalloc<unsigned>((size_t)W / 64 + 2);  // >= second-half workgroups / 128, zeroed
     d_count_ = dalloc<unsigned long long>(1);
@@ -645,8 +646,8 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     D.sort_cell = p.sort_cell_A > 0.0f ? p.sort_cell_A : 2.0f;
     // tiles per nonbonded unit: short units balance the persistent grid when
     // units are few, long ones save per-unit start-up when they are many
-    // (measured best: 10 at 25k atoms, 16 at 250k)
-    D.chunk = p.chunk_tiles > 0 ? p.chunk_tiles : (t->n_atoms < 100000 ? 10 : 24);
+    // (measured best with units claimed at run time: 8 at 25k atoms, 24 at 250k)
+    D.chunk = p.chunk_tiles > 0 ? p.chunk_tiles : (t->n_atoms < 100000 ? 8 : 24);
     D.dt = (float)p.dt_ps;
     D.cutoff = (float)p.cutoff_A;
     D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);

@@ -84,6 +84,9 @@ struct DevPtrs {
     int *sweeps;                    // [waves][2]: most SHAKE / RATTLE sweeps so far, per group wave
                                     // (written only by that wave; reduced on the host)
     EngineCounters *counters;
+    unsigned *nb_claim;             // [NB_CLAIM_COUNTERS * NB_CLAIM_STRIDE] unit-claim counters of the
+                                    // nonbonded kernel (zero between launches: each launch's last
+                                    // claimer resets its counter)
     unsigned long long *step_ts;    // [TS_RING] device wall clock at the end of step k (slot k % TS_RING)
 };
 

@@ -130,35 +130,53 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     float4 *rpos = rpos_s[wave];
     float4 *rpar = rpar_s[wave];
     const int n_units = P.counters->n_units;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // Workgroups are dispatched round-robin over the XCDs (workgroup w on XCD
     // w % 8, each XCD with its own L2).  Each XCD takes one contiguous eighth
     // of the unit list -- blocks are space-filling-curve ordered, so that is a
     // compact region whose j-tiles stay in its L2 -- and its waves go round
     // robin over it.  (A shared claim counter was measured far slower:
     // device-scope atomics on one address serialise.)
-    int u0 = blockIdx.x * NB_WAVES + wave, u_end = n_units, stride = gridDim.x * NB_WAVES;
-    if (gridDim.x % N_XCD == 0) {
-        const int xcd = blockIdx.x % N_XCD;
-        const int span_u = (n_units + N_XCD - 1) / N_XCD;
-        u0 = xcd * span_u + (blockIdx.x / N_XCD) * NB_WAVES + wave;
-        u_end = min(n_units, (xcd + 1) * span_u);
-        stride = (gridDim.x / N_XCD) * NB_WAVES;
-    }
-    // The codes of the wave's next 64 units arrive in one load, one per lane
-    // (a wave seldom runs more than a few units), and each unit reads its code
-    // from its lane.  The inner loop issues no load that a later unit waits on,
-    // so a unit never waits for the previous unit's force atomics to retire.
-    for (int ub = u0; ub < u_end; ub += 64 * stride) {
-        const int uu = ub + lane * stride;
-        const int codes = uu < u_end ? P.units[uu] : 0;
-        const int n_here = min(64, (u_end - ub + stride - 1) / stride);
-        // read before the loop, so the wait for the codes happens here and
-        // not at the loop head, where it would also wait for the atomics
-        const int code0 = __builtin_amdgcn_readlane(codes, 0);
-        for (int it = 0; it < n_here; ++it) {
-            const int u = ub + it * stride;
-            const int code = it == 0 ? code0 : __builtin_amdgcn_readlane(codes, it);
+    const bool by_xcd = gridDim.x % N_XCD == 0;
+    const int xcd = by_xcd ? (int)(blockIdx.x % N_XCD) : 0;
+    const int span_u = by_xcd ? (n_units + N_XCD - 1) / N_XCD : n_units;
+    const int u_begin = xcd * span_u, u_end = min(n_units, u_begin + span_u);
+    const int wave_x = (by_xcd ? (int)(blockIdx.x / N_XCD) : (int)blockIdx.x) * NB_WAVES + wave;  // wave within the range
+    const int waves_x = (by_xcd ? (int)(gridDim.x / N_XCD) : (int)gridDim.x) * NB_WAVES;
+    // Units are claimed, not assigned: the range is split into
+    // NB_CLAIM_GROUPS interleaved sub-lists (unit u_begin + g + k G), each
+    // with a counter that the range's waves of group g advance by one per
+    // claim, so a wave that drew short units draws more.  (The static
+    // round-robin it replaces left most waves idle behind the slowest.)  A
+    // unit is computed whole by the wave that claims it, so no result
+    // depends on the claim order.  Every wave of the group claims until a
+    // claim comes back empty; the group's last claim overall (value
+    // n_g + waves_g - 1) resets the counter to zero for the next launch.
+    const int G = min(NB_CLAIM_GROUPS, waves_x);  // every group has a wave
+    const int g = wave_x % G;
+    const int n_g = u_end > u_begin + g ? (u_end - u_begin - g + G - 1) / G : 0;
+    const int waves_g = (waves_x - g + G - 1) / G;
+    // (+ mbcnt(0) = 0: the compiler takes the address for a per-lane one and
+    // leaves the claim a plain returning atomic, whose result is first read
+    // at the unit's end; rewritten as a wave-aggregated atomic it would be
+    // read at once and the unit would start by waiting for it)
+    unsigned *ctr = P.nb_claim + (size_t)(xcd * G + g) * NB_CLAIM_STRIDE + __builtin_amdgcn_mbcnt_lo(0u, 0u);
+    // The next unit is claimed when a unit starts; the claim and the next
+    // unit's code are read at the unit's end, before its force atomics, so
+    // no unit start waits for a claim, a code or the previous atomics.
+    unsigned k_pending = 0u;
+    if (lane == 0) k_pending = atomicAdd(ctr, 1u);
+    int k_cur = __builtin_amdgcn_readfirstlane((int)k_pending);
+    int code_cur = k_cur < n_g ? __builtin_amdgcn_readfirstlane(P.units[u_begin + g + k_cur * G]) : 0;
+    for (;;) {
+        if (k_cur >= n_g) {
+            if (k_cur == n_g + waves_g - 1 && lane == 0) atomicExch(ctr, 0u);
+            break;
+        }
+        const int u = u_begin + g + k_cur * G;
+        const int code = code_cur;
+        if (lane == 0) k_pending = atomicAdd(ctr, 1u);
+        int next_code = 0;
+        {
             const int b = code >> 11, q = (code >> 9) & 3, t0 = code & (MAX_ROW - 1);
             // tile indices of up to CHUNK_MAX tiles, loaded before the row length
             // and the descriptor arrive (entries past the row are not used)
@@ -298,6 +316,8 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 e += __shfl_xor(e, 16);
                 e += __shfl_xor(e, 32);
             }
+            k_cur = __builtin_amdgcn_readfirstlane((int)k_pending);
+            if (k_cur < n_g) next_code = P.units[u_begin + g + k_cur * G];
             if (!DIRECT) {
                 if (js == 0) P.unit_f[(size_t)u * SUB + il] = make_float4(fx, fy, fz, ENERGY ? e : 0.0f);
             } else {
@@ -325,6 +345,7 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 }
             }
         }
+        code_cur = __builtin_amdgcn_readfirstlane(next_code);
     }
 }
 

@@ -26,6 +26,9 @@ constexpr int CHUNK_MIN = 8;         // tiles per nonbonded work unit: per syste
 constexpr int CHUNK_MAX = 32;        // [CHUNK_MIN, CHUNK_MAX] (SysDesc::chunk)
 constexpr int CHUNKS_PER_ROW = MAX_ROW / CHUNK_MIN;
 constexpr int UNITS_PER_BLOCK = QUARTERS * CHUNKS_PER_ROW;
+constexpr int NB_CLAIM_GROUPS = 8;   // unit sub-lists per XCD range, each with its claim counter
+constexpr int NB_CLAIM_COUNTERS = 8 * NB_CLAIM_GROUPS;
+constexpr int NB_CLAIM_STRIDE = 32;  // counters 128 bytes apart
 constexpr int FORCE_FIXED_BITS = 24; // int64 fixed point: 2^24 per kcal/mol/A
 constexpr int ENERGY_FIXED_BITS = 24;
 constexpr int TERM_CHUNK = 256;     // terms per live flag (term_live)

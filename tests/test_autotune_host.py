@@ -45,7 +45,7 @@ def test_tune_picks_the_fastest_candidate(monkeypatch):
         def step(self, n, graph_steps):
             # cost minimal at (2.0 A, 8, 24 tiles): |cell - 2| + |log2(resort) - 3| + |chunk - 24| / 100
             import math
-            chunk = getattr(self.p, "chunk_tiles", 0) or 10  # the engine's default below 100,000 atoms
+            chunk = getattr(self.p, "chunk_tiles", 0) or 8  # the engine's default below 100,000 atoms
             cost = 1.0 + abs(self.p.sort_cell_A - 2.0) + abs(math.log2(self.p.resort_every) - 3.0) + \
                 abs(chunk - 24) / 100.0
             clock["t"] += n * cost * 1e-6

@@ -27,6 +27,7 @@ import os
 
 import numpy as np
 import pytest
+from stepcheck import compare_step
 
 pytestmark = pytest.mark.gpu
 
@@ -204,9 +205,7 @@ def test_pool_slot_forces_and_step_match_oracle(oracle, served):
     pool.step(1, 0)
     xg, vg = pool.get_positions(slot), pool.get_velocities(slot)
     xo, vo, _ = oracle.run(top.with_positions(xs), p, xs, vs, step0, 1)
-    dx, dv = np.abs(xg - xo).max(), rel_rms(vg.astype(np.float64), vo)
-    print(f"[pool] slot {slot} step {step0}: max|dx| {dx:.2e} A, v rel RMS {dv:.2e}", flush=True)
-    assert dx <= 2e-5 and dv <= 1e-4
+    compare_step(top, p, xg, vg, xo, vo, 2e-5, 1e-4, f"pool slot {slot} step {step0}")
 
 
 @pytest.mark.timeout(600)

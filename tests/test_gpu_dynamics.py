@@ -60,26 +60,49 @@ def nve_start(pm):
     return top, x, pm.maxwell_boltzmann(top["masses"], 100.0, 3)
 
 
-def test_nve_energy_conservation(pm, nve_start):
-    """gamma = 0 selects exact kicks (no force clamp).  Over 5 ps at 0.5 fs the
-    total energy of the relaxed 2,000-atom chain stays within 0.75 % of the
-    kinetic energy, and doubling dt raises the mean error between 2.5- and
-    6-fold.  Both bars come from the FP64 oracle run from this same state
-    (same minimised positions and Maxwell-Boltzmann draw): its maximum error
-    at 0.5 fs is 0.48 % of the kinetic energy (3.98 kcal/mol; the GPU 4.05)
-    and its mean-error ratio 1 fs / 0.5 fs is 3.5 (the GPU 3.3); from a state
-    relaxed by the oracle itself the ratio is 2.6.  The error is the
-    algorithm's -- the potential-shift cutoff keeps the energy continuous but
-    not the force, which adds a part that is not second order in dt -- and a
-    force error of the kernels, independent of dt, would break the ratio."""
+def _oracle_series(pm, oracle, top, x, v, dt, chunks, chunk_ps):
+    p = pm.default_params(dt_ps=dt, gamma_ps=0.0, equilibration_steps=0, temperature_K=100.0,
+                          end_temperature_K=100.0, com_remove_every=0)
+    t = top.with_positions(x)
+    xo, vo, _ = oracle.run(t, p, x.astype(np.float64), v.astype(np.float64), 0, 1)
+    e0 = sum(oracle.energies(t, xo, vo))
+    out = []
+    for _ in range(chunks):
+        xo, vo, _ = oracle.run(t, p, xo, vo, 0, int(round(chunk_ps / dt)))
+        out.append(sum(oracle.energies(t, xo, vo)) - e0)
+    return np.array(out)
+
+
+def test_nve_energy_conservation(pm, oracle, nve_start):
+    """gamma = 0 selects exact kicks (no force clamp).
+
+    - Over 5 ps at 0.5 fs the total energy of the relaxed 2,000-atom chain
+      stays within 0.75 % of the kinetic energy (the FP64 oracle from this
+      state: 0.48 %).
+    - Over the first 50 fs, sampled every 10 fs, the energy error is the
+      integrator's shadow offset; it follows the FP64 oracle's from the same
+      state within 0.02 kcal/mol at 1 fs and 0.5 fs (measured 0.004; the
+      offsets are 1.6 and 0.4 kcal/mol), and doubling dt scales it 3- to
+      5.5-fold (second order: 4; measured 4.2, the oracle 4.0).  A force
+      error of the kernels would separate the series at once.
+    - The ratio is taken on this short window because over picoseconds the
+      potential-shift cutoff (continuous energy, discontinuous force) adds an
+      error that is not second order in dt: the FP64 oracle's own 5 ps
+      mean-error ratio 1 fs / 0.5 fs ranges from 1.6 to 10.5 over four
+      Maxwell-Boltzmann draws of this chain."""
     top, x, v = nve_start
     _, fine, ke = _nve_series(pm, top, x, v, 0.0005, 10, 0.5)
-    _, coarse, _ = _nve_series(pm, top, x, v, 0.001, 10, 0.5)
-    print(f"[nve] 0.5 fs: {np.round(fine, 2).tolist()}  1 fs: {np.round(coarse, 2).tolist()}  KE {ke:.0f}",
-          flush=True)
+    print(f"[nve] 0.5 fs, 5 ps: {np.round(fine, 2).tolist()}  KE {ke:.0f}", flush=True)
     assert np.abs(fine).max() < 0.0075 * ke
-    ratio = np.abs(coarse).mean() / np.abs(fine).mean()
-    assert 2.5 < ratio < 6.0, ratio
+    short = {}
+    for dt in (0.001, 0.0005):
+        _, g, _ = _nve_series(pm, top, x, v, dt, 5, 0.01)
+        o = _oracle_series(pm, oracle, top, x, v, dt, 5, 0.01)
+        print(f"[nve {dt * 1e3:g} fs, 50 fs] gpu {np.round(g, 4).tolist()} oracle {np.round(o, 4).tolist()}", flush=True)
+        assert np.abs(g - o).max() <= 0.02, (dt, g, o)
+        short[dt] = np.abs(g).mean()
+    ratio = short[0.001] / short[0.0005]
+    assert 3.0 < ratio < 5.5, ratio
 
 
 def test_nve_drift_matches_oracle(pm, oracle, nve_start):

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import numpy as np
 import pytest
+from stepcheck import compare_step
 
 pytestmark = pytest.mark.gpu
 
@@ -105,10 +106,7 @@ def test_integrator_explicit_noise_per_step(system, oracle):
         eng.step(1, noise=nz[k:k + 1])
         xg, vg = eng.get_positions(), eng.get_velocities()
         xo, vo, _ = oracle.run(top.with_positions(xs), eng.params, xs, vs, step0 + k, 1, nz[k:k + 1])
-        dx = np.abs(xg - xo).max()
-        dv = rel_rms(vg.astype(np.float64), vo)
-        print(f"[step {k}] max|dx| {dx:.2e} A, v rel RMS {dv:.2e}", flush=True)
-        assert dx <= 2e-5 and dv <= 1e-4
+        compare_step(top, eng.params, xg, vg, xo, vo, 2e-5, 1e-4, f"step {k}")
         xs, vs = xg, vg
 
 
