@@ -204,6 +204,19 @@ static PrismMdSlotCapsV1 exact_caps(const PrismMdTopologyV1 *t) {
     return c;
 }
 
+// Constraint groups of a topology: its clusters, then one per atom outside
+// every cluster (the grouping lower() builds).
+static int group_count(const PrismMdTopologyV1 *t) {
+    std::vector<char> in_cluster(t->n_atoms, 0);
+    for (int c = 0; c < t->n_clusters; ++c) {
+        in_cluster[t->cluster_central[c]] = 1;
+        for (int k = 0; k < t->cluster_n_hydrogens[c]; ++k) in_cluster[t->cluster_hydrogens[3 * c + k]] = 1;
+    }
+    int n = t->n_clusters;
+    for (int i = 0; i < t->n_atoms; ++i) n += in_cluster[i] ? 0 : 1;
+    return n;
+}
+
 static std::string fits(const PrismMdSlotCapsV1 &c, const PrismMdTopologyV1 *t) {
     if (t->n_atoms >= c.atoms) return "atoms";
     if (t->n_bonds > c.bonds) return "bonds";
@@ -214,7 +227,7 @@ static std::string fits(const PrismMdSlotCapsV1 &c, const PrismMdTopologyV1 *t) 
     return "";
 }
 
-int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
+int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream, const std::vector<int> &group_waves) {
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(PRISM_MD_E_NODEVICE, "no HIP device");
     if (stream) {
@@ -242,7 +255,9 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
         S.atom_base = A;
         S.atom_cap = pad64(c.atoms);
         S.wave_base = W;
-        S.wave_cap = S.atom_cap / 64;
+        // constraint-group waves: exact when the system is known at create
+        // (a created engine), else one group per atom at worst (a pool slot)
+        S.wave_cap = slots_.size() < group_waves.size() ? std::max(1, group_waves[slots_.size()]) : S.atom_cap / 64;
         S.bond_base = (int)NB; S.bond_cap = c.bonds;
         S.angle_base = (int)NA; S.angle_cap = c.angles;
         S.dih_base = (int)ND; S.dih_cap = c.dihedrals;
@@ -388,6 +403,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream) {
     P_.term_major = A >= 100000 ? 1 : 0;
     P_.n_bonds = (int)NB; P_.n_angles = (int)NA; P_.n_dih = (int)ND; P_.n_exc = (int)NE;
     P_.sys = d_desc_;
+    seam_ = seam_fits(P_);
     const std::vector<int> minus1(A, -1);
     int rc = PRISM_MD_OK;
     if ((rc = upload(P_.x, h.x)) || (rc = upload(P_.v, h.v)) || (rc = upload(d_mass_, h.mass)) ||
@@ -525,6 +541,7 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     // constraint groups: clusters, then lone atoms, then padding (topology
     // order keeps a wave's atoms close in memory; ordering the clusters by
     // hydrogen count to give waves one solver shape was measured slower)
+    if (group_count(t) > L.wave_cap * 64) return fail(PRISM_MD_E_CAPACITY, "system does not fit its slot: constraint groups");
     std::vector<char> in_cluster(n, 0);
     int ng = 0;
     auto put_group = [&](const Group &G, const GroupParam &Q) {
@@ -744,12 +761,14 @@ int Engine::upload_slot(int s) {
 
 int Engine::init(const std::vector<SystemSpec> &systems, void *stream) {
     std::vector<PrismMdSlotCapsV1> caps;
+    std::vector<int> waves;
     for (const auto &sp : systems) {
         std::string why;
         if (validate(sp.topo, sp.params, why) != PRISM_MD_OK) return fail(PRISM_MD_E_INVALID, "invalid topology/params: " + why);
         caps.push_back(exact_caps(sp.topo));
+        waves.push_back((group_count(sp.topo) + 63) / 64);
     }
-    if (int rc = layout(caps, stream)) return rc;
+    if (int rc = layout(caps, stream, waves)) return rc;
     for (int s = 0; s < (int)systems.size(); ++s) {
         if (int rc = lower(s, systems[s].topo, systems[s].params)) return rc;
         if (int rc = upload_slot(s)) return rc;
@@ -853,6 +872,64 @@ void Engine::launch_step(const float *noise_n, const float *noise_np1, bool samp
     ++force_evals_;
 }
 
+// The rest of a step whose first half has run: the rebuild pair, the forces,
+// then either the seam, which also runs the next step's first half, or the
+// second half alone.
+void Engine::launch_tail(bool seam, bool sample) {
+    launch_sort(P_, true, stream_);
+    launch_rows(P_, stream_);
+    launch_forces(true, sample);
+    if (seam) launch_seam(P_, stream_);
+    else launch_second_half(P_, nullptr, stream_);
+    ++force_evals_;
+}
+
+// The graph and eager unit of a run: a whole step, or with the seam a step
+// tail that both starts and ends inside a seam.
+void Engine::launch_unit() {
+    if (seam_) launch_tail(true);
+    else launch_step(nullptr, nullptr);
+}
+
+// n_steps steps without explicit noise.  With the seam (seam_fits: a
+// four-lane arena whose integrator workgroups are all resident at once) each
+// step of the call but the last ends in a seam launch, so the call opens with
+// one first half and closes with one second half; otherwise every step is the
+// five launches of launch_step.  Graphs of graph_steps units replay the bulk.
+// With nonbonded sampling on (`sample`, kernel_timing), every
+// timing_every-th step runs eagerly with its nonbonded launch bracketed by
+// events, and the units between two sampled steps replay as one graph.
+int Engine::run_steps(uint64_t n_steps, int graph_steps, bool sample) {
+    if (n_steps == 0) return PRISM_MD_OK;
+    const uint64_t period = sample && timing_every_ > 0 ? (uint64_t)timing_every_ : 0;
+    if (!seam_) {
+        if (!period) return replay(n_steps, graph_steps);
+        for (uint64_t done = 0; done < n_steps;) {
+            launch_step(nullptr, nullptr, true);  // sampled
+            const uint64_t rest = std::min<uint64_t>(period - 1, n_steps - done - 1);
+            if (rest)
+                if (int rc = replay(rest, (int)(period - 1))) return rc;
+            done += 1 + rest;
+        }
+        return PRISM_MD_OK;
+    }
+    launch_first_half(P_, nullptr, stream_);
+    for (uint64_t done = 0; done < n_steps;) {  // done: step tails launched
+        const bool sampled = period && done % period == 0;
+        if (sampled || done + 1 == n_steps) {
+            launch_tail(done + 1 < n_steps, sampled);
+            ++done;
+            continue;
+        }
+        // the unsampled tails before the next sampled one or the last
+        uint64_t upto = n_steps - 1;
+        if (period) upto = std::min(upto, (done / period + 1) * period);
+        if (int rc = replay(upto - done, period ? (int)(period - 1) : graph_steps)) return rc;
+        done = upto;
+    }
+    return PRISM_MD_OK;
+}
+
 // Turn the device's step-end stamps since the last harvest into per-step
 // latencies of every system that advanced.  Systems only stop (limit, fault)
 // or start (admission, activation) between host calls, so a system that
@@ -953,6 +1030,7 @@ int Engine::check_status() {
         if (b & ST_EXCEPTION_COINCIDENT) return fail(PRISM_MD_E_NONFINITE, "coincident exception atoms" + where);
         if (b & ST_ROW_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "tile row overflow" + where);
         if (b & ST_MASK_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "exclusion-mask overflow" + where);
+        if (b & ST_BARRIER) return fail(PRISM_MD_E_HIP, "step seam: grid barrier timed out" + where);
         return fail(PRISM_MD_E_INVALID, "device status " + std::to_string(b) + where);
     }
     return PRISM_MD_OK;
@@ -1052,26 +1130,15 @@ int Engine::compute_forces(unsigned mask, int s, float *F_out, int64_t *F_fixed,
     return check_status();
 }
 
-// Steps without explicit noise replay captured graphs of whole steps.  With
+// Steps without explicit noise replay captured graphs (run_steps).  With
 // nonbonded sampling on (kernel_timing), each sampling period is one eager
 // step whose nonbonded launch is bracketed by events, then a graph of the
 // period's other steps.  Explicit noise is staged per step, so those steps
-// run eagerly.
+// run eagerly, each as its own five launches.
 int Engine::step(uint64_t n_steps, const float *noise) {
     if (noise && sys_.size() != 1) return fail(PRISM_MD_E_UNSUPPORTED, "explicit noise needs a single system");
     if (!noise) {
-        if (timing_every_ <= 0) {
-            if (int rc = replay(n_steps, SINGLE_GRAPH_STEPS)) return rc;
-        } else {
-            const uint64_t period = (uint64_t)timing_every_;
-            for (uint64_t done = 0; done < n_steps;) {
-                launch_step(nullptr, nullptr, true);  // sampled
-                const uint64_t rest = std::min<uint64_t>(period - 1, n_steps - done - 1);
-                if (rest)
-                    if (int rc = replay(rest, (int)(period - 1))) return rc;
-                done += 1 + rest;
-            }
-        }
+        if (int rc = run_steps(n_steps, SINGLE_GRAPH_STEPS, true)) return rc;
         host_steps_ += n_steps;
         if (int rc = hipcheck(hipGetLastError(), "step launch")) return rc;
         const int rc = check_status();
@@ -1128,9 +1195,9 @@ int Engine::kernel_timing(int every, double *mean_ms, uint64_t *samples) {
     return PRISM_MD_OK;
 }
 
-// n_steps steps as replays of a captured graph of graph_steps steps (captured
-// once per length and kept), the remainder eagerly.  No event records are
-// captured.
+// n_steps units (launch_unit) as replays of a captured graph of graph_steps
+// units (captured once per length and kept), the remainder eagerly.  No event
+// records are captured.
 int Engine::replay(uint64_t n_steps, int graph_steps) {
     if (graph_steps < 1) return fail(PRISM_MD_E_INVALID, "graph_steps");
     hipGraphExec_t exec = nullptr;
@@ -1149,7 +1216,7 @@ int Engine::replay(uint64_t n_steps, int graph_steps) {
             timing_every_ = every;
             return rc;
         }
-        for (int k = 0; k < graph_steps; ++k) launch_step(nullptr, nullptr);
+        for (int k = 0; k < graph_steps; ++k) launch_unit();
         const hipError_t ce = hipStreamEndCapture(stream_, &g);
         timing_every_ = every;
         force_evals_ = evals;
@@ -1164,13 +1231,13 @@ int Engine::replay(uint64_t n_steps, int graph_steps) {
         for (; done + graph_steps <= n_steps; done += graph_steps)
             if (int rc = hipcheck(hipGraphLaunch(exec, stream_), "graph launch")) return rc;
     force_evals_ += done;
-    for (; done < n_steps; ++done) launch_step(nullptr, nullptr);  // counts its own evaluation
+    for (; done < n_steps; ++done) launch_unit();  // counts its own evaluation
     return PRISM_MD_OK;
 }
 
 int Engine::step_graph(uint64_t n_steps, int graph_steps) {
     if (graph_steps < 1) return fail(PRISM_MD_E_INVALID, "graph_steps");
-    if (int rc = replay(n_steps, graph_steps)) return rc;
+    if (int rc = run_steps(n_steps, graph_steps, false)) return rc;
     host_steps_ += n_steps;
     if (int rc = hipcheck(hipGetLastError(), "graph step")) return rc;
     return check_status();
@@ -1441,7 +1508,7 @@ int Engine::set_active(int s, bool active) {
         // drop its units: every active block re-claims, no row changes
         c.any_dirty = 1;
         // the flags only: the step-end count behind them keeps running
-        if (int rc = hipcheck(hipMemcpyAsync(P_.counters, &c, offsetof(EngineCounters, pad), hipMemcpyHostToDevice, stream_),
+        if (int rc = hipcheck(hipMemcpyAsync(P_.counters, &c, offsetof(EngineCounters, seam_gen), hipMemcpyHostToDevice, stream_),
                               "counters"))
             return rc;
         launch_rows(P_, stream_);
@@ -1578,7 +1645,7 @@ int Engine::checkpoint_restore(int s, const void *buf, size_t len) {
     EngineCounters c{};
     c.any_dirty = 1;
     if ((rc = hipcheck(hipMemcpyAsync(P_.sst + s, &st, sizeof st, hipMemcpyHostToDevice, stream_), "restore state")) ||
-        (rc = hipcheck(hipMemcpyAsync(P_.counters, &c, offsetof(EngineCounters, pad), hipMemcpyHostToDevice, stream_),
+        (rc = hipcheck(hipMemcpyAsync(P_.counters, &c, offsetof(EngineCounters, seam_gen), hipMemcpyHostToDevice, stream_),
                        "restore counters")))
         return rc;
     launch_rows(P_, stream_);

@@ -100,11 +100,15 @@ class Engine {
     int emit_pairs(int s, int mode, int *i_d, int *j_d, unsigned long long *count_d, size_t cap);
     int refresh_all();            // rebuild lists + forces at current positions
     int refresh(int only);        // the same for one system (-1: all)
-    int layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream);
+    int layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream, const std::vector<int> &group_waves = {});
     int lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p);
     int upload_slot(int s);
     template <class T> int upload_range(T *d, const std::vector<T> &h, size_t begin, size_t count);
     void launch_step(const float *noise_n, const float *noise_np1, bool sample = false);
+    void launch_tail(bool seam, bool sample = false);
+    void launch_unit();
+    int run_steps(uint64_t n_steps, int graph_steps, bool sample);
+    bool seam_ = false;           // fused step seams (launch_seam) usable for this arena
     void full_force();
     void harvest_timing();
 

@@ -118,6 +118,10 @@ void launch_nonbonded_direct(const DevPtrs &P, hipStream_t s, hipEvent_t start =
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);
 void launch_second_half(const DevPtrs &P, const float *noise, hipStream_t s);
+// The second half of a step and the first half of the next in one launch
+// (no explicit noise); usable when seam_fits(P): every workgroup resident.
+bool seam_fits(const DevPtrs &P);
+void launch_seam(const DevPtrs &P, hipStream_t s);
 void launch_zero_forces(const DevPtrs &P, hipStream_t s);
 // Sum the nonbonded unit partials and the bonded slots of every atom, in fixed
 // order, into the int64 fixed-point accumulator (one writer per atom).

@@ -91,36 +91,81 @@ __device__ inline bool solve3(int k, double M[3][3], double b[3], double lam[3])
 // those conditions are linear in the multipliers, so here the k x k system
 //   sum_j [m_c^-1 (r_h . r_j) + m_h^-1 delta_hj (r_h . r_h)] lam_j = r_h . (v_h - v_c)
 // is solved exactly in FP64: the fixed point of that iteration in one step.
-// Returns 1, or -1 when the cluster is degenerate or the result misses tol.
-__device__ int rattle(GroupState &g, const GroupParam &gp, double tol, int max_iter) {
-    (void)max_iter;
-    if (g.n < 2) return 0;
+//
+// The matrix depends on the positions only, and the two RATTLE projections
+// of each half step see the same positions, so the solve is split: a factor
+// (the bond vectors and the cofactors of solve3's padded 3 x 3 system) once
+// per half step, then one application per projection -- the same operations
+// on the same operands as solve3, so bitwise the same multipliers.
+struct RattleFactor {
+    double r[3][3];  // bond vectors x_h - x_c
+    double c[3][3];  // c[i][j]: cofactor (i, j) of the padded matrix
+    double id;       // 1 / det
+    bool ok;         // det nonzero
+};
+
+__device__ inline void rattle_factor(const GroupState &g, const GroupParam &gp, RattleFactor &F) {
     const int k = g.n - 1;
     const double imc = gp.inv_mass_c, imh = gp.inv_mass_h;
-    double r[3][3] = {}, u[3][3] = {}, M[3][3] = {}, b[3] = {}, lam[3] = {};
+    double M[3][3] = {};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 3; ++c) F.r[i][c] = 0.0;
     FOR_H(h) {
-        for (int c = 0; c < 3; ++c) {
-            r[h - 1][c] = (double)g.x[h][c] - (double)g.x[0][c];
-            u[h - 1][c] = (double)g.v[h][c] - (double)g.v[0][c];
-        }
+        for (int c = 0; c < 3; ++c) F.r[h - 1][c] = (double)g.x[h][c] - (double)g.x[0][c];
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         if (i < k) {
-            b[i] = r[i][0] * u[i][0] + r[i][1] * u[i][1] + r[i][2] * u[i][2];
 #pragma unroll
             for (int j = 0; j < 3; ++j)
                 if (j < k) {
-                    const double rr = r[i][0] * r[j][0] + r[i][1] * r[j][1] + r[i][2] * r[j][2];
+                    const double rr = F.r[i][0] * F.r[j][0] + F.r[i][1] * F.r[j][1] + F.r[i][2] * F.r[j][2];
                     M[i][j] = imc * rr + (i == j ? imh * rr : 0.0);
                 }
         }
-    if (!solve3(k, M, b, lam)) return -1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (i >= k) {
+            for (int j = 0; j < 3; ++j) M[i][j] = M[j][i] = 0.0;
+            M[i][i] = 1.0;
+        }
+    const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+    const double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+    const double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+    const double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+    F.ok = fabs(det) > 1e-300;
+    F.id = 1.0 / det;
+    F.c[0][0] = c00; F.c[0][1] = c01; F.c[0][2] = c02;
+    F.c[1][0] = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+    F.c[1][1] = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+    F.c[1][2] = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+    F.c[2][0] = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+    F.c[2][1] = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+    F.c[2][2] = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+}
+
+// Returns 1, or -1 when the cluster is degenerate or the result misses tol.
+__device__ int rattle_apply(GroupState &g, const GroupParam &gp, const RattleFactor &F, double tol) {
+    if (g.n < 2) return 0;
+    const int k = g.n - 1;
+    const double imc = gp.inv_mass_c, imh = gp.inv_mass_h;
+    double u[3][3] = {}, b[3] = {}, lam[3] = {};
+    FOR_H(h) {
+        for (int c = 0; c < 3; ++c) u[h - 1][c] = (double)g.v[h][c] - (double)g.v[0][c];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (i < k) b[i] = F.r[i][0] * u[i][0] + F.r[i][1] * u[i][1] + F.r[i][2] * u[i][2];
+    if (!F.ok) return -1;
+    lam[0] = (F.c[0][0] * b[0] + F.c[1][0] * b[1] + F.c[2][0] * b[2]) * F.id;
+    lam[1] = (F.c[0][1] * b[0] + F.c[1][1] * b[1] + F.c[2][1] * b[2]) * F.id;
+    lam[2] = (F.c[0][2] * b[0] + F.c[1][2] * b[1] + F.c[2][2] * b[2]) * F.id;
     double dv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < 3; ++j)
         if (j < k)
-            for (int c = 0; c < 3; ++c) dv[c] += lam[j] * r[j][c];
+            for (int c = 0; c < 3; ++c) dv[c] += lam[j] * F.r[j][c];
     double vc[3];
     for (int c = 0; c < 3; ++c) vc[c] = (double)g.v[0][c] + imc * dv[c];
     // residual test |r.u| / (|r| max(|u|, 1)) <= tol, squared so that the
@@ -129,10 +174,11 @@ __device__ int rattle(GroupState &g, const GroupParam &gp, double tol, int max_i
     bool bad = false, ok = true;
     FOR_H(h) {
         double vh[3];
-        for (int c = 0; c < 3; ++c) vh[c] = (double)g.v[h][c] - imh * lam[h - 1] * r[h - 1][c];
+        for (int c = 0; c < 3; ++c) vh[c] = (double)g.v[h][c] - imh * lam[h - 1] * F.r[h - 1][c];
         const double ux = vh[0] - vc[0], uy = vh[1] - vc[1], uz = vh[2] - vc[2];
-        const double r2 = r[h - 1][0] * r[h - 1][0] + r[h - 1][1] * r[h - 1][1] + r[h - 1][2] * r[h - 1][2];
-        const double res = r[h - 1][0] * ux + r[h - 1][1] * uy + r[h - 1][2] * uz;
+        const double *r = F.r[h - 1];
+        const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        const double res = r[0] * ux + r[1] * uy + r[2] * uz;
         const double lhs = res * res, rhs = t2 * r2 * fmax(ux * ux + uy * uy + uz * uz, 1.0);
         bad |= !(r2 >= 1e-12) || !isfinite(lhs) || !isfinite(rhs);
         ok &= lhs <= rhs;
@@ -246,15 +292,19 @@ __device__ inline double ke_change(double m, double before2, double after2) {
 // layouts below call these, so they give bitwise the same trajectory.
 
 // langevin_thermostat with dt/2 (nhs_amber_fused.cu:1713-1744, :3656-3666)
-// of atom a; returns its kinetic-energy change for the work ledger.
-__device__ inline double ou_atom(float v[3], float m, int a, const SysDesc &D, float c1, float T,
-                                 unsigned long long step, int purpose, const float *noise) {
-    float xi[3];
+// of atom a, in two parts: the normals, which depend on the atom, the step
+// and the phase only (so a kernel can draw them while its loads are in
+// flight), then the update; returns its kinetic-energy change for the work
+// ledger.  `a` may be -1 (no atom: the draw is then never used).
+__device__ inline void ou_noise(float xi[3], int a, const SysDesc &D, unsigned long long step, int purpose,
+                                const float *noise) {
     if (noise) {
-        for (int c = 0; c < 3; ++c) xi[c] = noise[(size_t)a * 3 + c];
+        for (int c = 0; c < 3; ++c) xi[c] = noise[(size_t)max(a, 0) * 3 + c];
     } else {
         langevin_normals(D.seed_lo, D.stream, (uint32_t)(a - D.atom_begin), step, purpose, xi);
     }
+}
+__device__ inline double ou_apply(float v[3], float m, float c1, float T, const float xi[3]) {
     // hardware reciprocal and square root (~1 ulp): the IEEE-rounded forms
     // cost ~10 dependent instructions each on a latency-bound lane
     const float c2 = __builtin_amdgcn_sqrtf((1.0f - c1 * c1) * (float)KCAL_TO_AMU_A2_PS2 * (float)KB * T *
@@ -262,6 +312,12 @@ __device__ inline double ou_atom(float v[3], float m, int a, const SysDesc &D, f
     const double before = speed2(v);
     for (int c = 0; c < 3; ++c) v[c] = c1 * v[c] + c2 * xi[c];
     return ke_change(m, before, speed2(v));
+}
+__device__ inline double ou_atom(float v[3], float m, int a, const SysDesc &D, float c1, float T,
+                                 unsigned long long step, int purpose, const float *noise) {
+    float xi[3];
+    ou_noise(xi, a, D, step, purpose, noise);
+    return ou_apply(v, m, c1, T, xi);
 }
 
 // Half kick of one atom with the 1000 kcal/mol/A clamp of
@@ -356,10 +412,10 @@ __device__ inline bool load_group(const DevPtrs &P, int gidx, GroupState &g, Gro
 
 // RATTLE whose kinetic-energy change goes to the work ledger (the bath-side
 // projections, see group_ke2).
-__device__ inline int rattle_booked(GroupState &g, const GroupParam &gp, double tol, int max_iter) {
+__device__ inline int rattle_booked(GroupState &g, const GroupParam &gp, const RattleFactor &F, double tol) {
     if (g.n < 2) return 0;
     const double before = group_ke2(g, gp);
-    const int r = rattle(g, gp, tol, max_iter);
+    const int r = rattle_apply(g, gp, F, tol);
     g.w += 0.5 * (group_ke2(g, gp) - before) * (1.0 / KCAL_TO_AMU_A2_PS2);
     return r;
 }
@@ -378,13 +434,12 @@ __device__ inline void note_sweeps(SysState &S, int sweeps, int kind, int atom, 
     if (sweeps < 0) record_fault(S, kind, atom, step);
     else most = max(most, sweeps);
 }
-// Sweep statistics: the wave's maximum raises the running maximum `prev`
-// (read at kernel start) of its group wave's slot, which the four kernel
-// waves of the four-lane layout share -- hence the atomic, issued only when
-// it raises.  Called in converged control flow.
+// Sweep statistics: a lane whose count exceeds the running maximum `prev`
+// (read at kernel start) of its group wave's slot raises it with an atomic
+// that nothing waits for -- rare once the counts have settled, and cheaper on
+// the lane's dependent chain than a reduction over the wave first.
 __device__ inline void publish_sweeps(int *slot, int prev, int most) {
-    for (int o = 32; o > 0; o >>= 1) most = max(most, __shfl_xor(most, o));
-    if ((threadIdx.x & 63) == 0 && most > prev) atomicMax(slot, most);
+    if (most > prev) atomicMax(slot, most);
 }
 
 // A coordinate beyond 1000 A means the system has diverged (heartbeat_check,
@@ -430,13 +485,10 @@ __device__ inline double quad_lane(double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// The quad's positions and velocities into every lane's group copy.
-__device__ inline void quad_gather_xv(GroupState &g, const float x[3], const float v[3]) {
-#define PRISM_QG(J)                                \
-    for (int c = 0; c < 3; ++c) {                  \
-        g.x[J][c] = quad_lane<J>(x[c]);            \
-        g.v[J][c] = quad_lane<J>(v[c]);            \
-    }
+// The quad's positions or velocities into every lane's group copy.
+__device__ inline void quad_gather_x(GroupState &g, const float x[3]) {
+#define PRISM_QG(J) \
+    for (int c = 0; c < 3; ++c) g.x[J][c] = quad_lane<J>(x[c]);
     PRISM_QG(0) PRISM_QG(1) PRISM_QG(2) PRISM_QG(3)
 #undef PRISM_QG
 }
@@ -525,13 +577,15 @@ __global__ __launch_bounds__(64 * IW) void k_first_half1(DevPtrs P, const float 
             const float c1 = expf(-gamma * (0.5f * D.dt));
             FOR_ATOMS(k) g.w += ou_atom(g.v[k], g.m[k], g.a[k], D, c1, T, step, 1, noise);
         }
-        note_sweeps(S, rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_1, g.a[0], step, most_ra);
+        RattleFactor F;
+        rattle_factor(g, gp, F);
+        note_sweeps(S, rattle_booked(g, gp, F, D.rattle_tol), FAULT_RATTLE_1, g.a[0], step, most_ra);
         FOR_ATOMS(k) {
             freeze_atom(P, g.a[k]);  // the step's force evaluation adds into the zeroed accumulator
             double dw;
             if (kick_atom(g.v[k], g.f[k], g.m[k], D.dt, gamma, dw)) g.w += dw;
         }
-        note_sweeps(S, rattle(g, gp, D.rattle_tol, D.constraint_max_iter), FAULT_RATTLE_2, g.a[0], step, most_ra);
+        note_sweeps(S, rattle_apply(g, gp, F, D.rattle_tol), FAULT_RATTLE_2, g.a[0], step, most_ra);
         float xs[4][3];
         FOR_ATOMS(k) {
             for (int c = 0; c < 3; ++c) {
@@ -564,56 +618,98 @@ __global__ __launch_bounds__(64 * IW) void k_first_half1(DevPtrs P, const float 
 }
 
 // Layout 4: kernel wave kw covers groups 16 (kw & 3) .. 16 (kw & 3) + 15 of
-// group wave kw >> 2; a workgroup is one group wave.
-__global__ __launch_bounds__(64 * IW) void k_first_half4(DevPtrs P, const float *noise) {
-    const int lane = threadIdx.x & 63, k = lane & 3;
-    const int kw = blockIdx.x * IW + wave_id();
-    const int gw = kw >> 2;
-    if (gw >= P.n_waves_arena) return;
-    const int gidx = gw * 64 + (kw & 3) * 16 + (lane >> 2);
-    // the group's records load before the system's: they depend on nothing
-    // that the system's records decide
-    const int a = P.groups[gidx].atom[k];  // this lane's atom, -1 if the group has none here
-    const GroupParam gp = P.gparams[gidx];
-    const int s = P.wave_sys[gw];
-    if (s < 0) return;
-    const SysDesc &D = P.sys[s];
-    if (!D.active) return;
-    SysState &S = P.sst[s];
-    if (S.halted || parked(D, S)) {
+// group wave kw >> 2; a workgroup is one group wave.  The phases are
+// functions of what one lane holds (QuadAtom) so that the separate halves
+// and the fused step seam (k_seam4) run the same code.
+
+// One lane's atom as both halves read it: the group record and parameters,
+// the atom's state and carried force, its sorted slot and build position, the
+// group's cumulative work (quad-uniform) and the group wave's sweep maxima.
+struct QuadAtom {
+    int a;  // this lane's atom, -1 if the group has none here
+    GroupParam gp;
+    float m;
+    float x[3], v[3];
+    long long fr[3];
+    int slot;
+    float xb[3];
+    double w;
+    int prev_sh, prev_ra;
+};
+
+// Every load of a lane in one round after the group record: an absent atom
+// reads arena atom 0, whose values are never used.
+template <bool TAIL>
+__device__ inline void load_quad(const DevPtrs &P, int gw, int gidx, int k, QuadAtom &q) {
+    q.a = P.groups[gidx].atom[k];
+    q.gp = P.gparams[gidx];
+    const size_t ac = (size_t)max(q.a, 0);
+    q.m = P.mass[ac];
+    const float4 xa = P.x[ac], va = P.v[ac];
+    q.x[0] = xa.x; q.x[1] = xa.y; q.x[2] = xa.z;
+    q.v[0] = va.x; q.v[1] = va.y; q.v[2] = va.z;
+    for (int c = 0; c < 3; ++c) q.fr[c] = P.f[ac * 4 + c];
+    if (TAIL) {
+        q.slot = P.slot_of[ac];
+        const float4 b = P.x_build[ac];
+        q.xb[0] = b.x; q.xb[1] = b.y; q.xb[2] = b.z;
+    }
+    q.w = P.work_g[gidx];
+    q.prev_sh = P.sweeps[2 * gw];
+    q.prev_ra = P.sweeps[2 * gw + 1];
+}
+
+// The fields of a system's step state that the first half reads.
+struct StepView {
+    unsigned long long step;
+    long long rebuild_step;
+    int halted;
+    double com_v[3];
+};
+__device__ inline StepView step_view(const SysState &S) {
+    StepView V;
+    V.step = S.step;
+    V.rebuild_step = S.rebuild_step;
+    V.halted = S.halted;
+    for (int c = 0; c < 3; ++c) V.com_v[c] = S.com_v[c];
+    return V;
+}
+
+// First half of one lane's atom (the step's opening phases, see above).
+__device__ inline void first_half4(const DevPtrs &P, const float *noise, int gw, int gidx, int lane, const SysDesc &D,
+                                   SysState &S, const StepView &V, const QuadAtom &q) {
+    const int k = lane & 3, a = q.a;
+    if (V.halted || (D.end_step != 0 && V.step >= D.end_step)) {  // halted or parked (see parked())
         freeze_atom(P, a);
         return;
     }
-    const size_t ac = (size_t)max(a, 0);  // an absent atom reads arena atom 0, never used
-    const float m = P.mass[ac];
-    const float4 xa = P.x[ac], va = P.v[ac];
-    long long fr[3];
-    for (int c = 0; c < 3; ++c) fr[c] = P.f[ac * 4 + c];
-    const int slot = P.slot_of[ac];
-    const float4 xb = P.x_build[ac];
-    const unsigned long long step = S.step;
+    const unsigned long long step = V.step;
     float T, gamma;
     schedule(D, step + 1, T, gamma);
     const bool com = D.com_every > 0 && step > 0 && step % (unsigned long long)D.com_every == 0;
-    const int prev_sh = P.sweeps[2 * gw], prev_ra = P.sweeps[2 * gw + 1];
     GroupState g;
-    g.w = P.work_g[gidx];
+    g.w = q.w;
     bool moved = false;
     int most_sh = 0, most_ra = 0;
     if (quad_lane<0>(a) >= 0) {  // quad-uniform: not a padding group
-        g.n = 1 + gp.n_h;
+        g.n = 1 + q.gp.n_h;
         const bool mine = k < g.n;
-        float x[3] = {xa.x, xa.y, xa.z}, v[3] = {va.x, va.y, va.z};
+        float x[3] = {q.x[0], q.x[1], q.x[2]}, v[3] = {q.v[0], q.v[1], q.v[2]};
+        float xi[3] = {0.0f, 0.0f, 0.0f};
+        if (gamma != 0.0f) ou_noise(xi, a, D, step, 1, noise);
+        quad_gather_x(g, x);
+        RattleFactor F;  // both projections below see these positions
+        rattle_factor(g, q.gp, F);
         if (com)
-            for (int c = 0; c < 3; ++c) v[c] -= (float)S.com_v[c];
+            for (int c = 0; c < 3; ++c) v[c] -= (float)V.com_v[c];
         if (gamma != 0.0f) {
             const float c1 = expf(-gamma * (0.5f * D.dt));
             double dw = 0.0;
-            if (mine) dw = ou_atom(v, m, a, D, c1, T, step, 1, noise);
+            if (mine) dw = ou_apply(v, q.m, c1, T, xi);
             quad_book(g.w, dw, mine);
         }
-        quad_gather_xv(g, x, v);
-        int r = rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter);
+        quad_gather_v(g, v);
+        int r = rattle_booked(g, q.gp, F, D.rattle_tol);
         if (k == 0) note_sweeps(S, r, FAULT_RATTLE_1, a, step, most_ra);
         PRISM_OWN_ROW(g.v, k, v);
         {
@@ -621,12 +717,12 @@ __global__ __launch_bounds__(64 * IW) void k_first_half4(DevPtrs P, const float 
             bool hit = false;
             if (mine) {
                 freeze_atom(P, a);
-                hit = kick_atom(v, fr, m, D.dt, gamma, dw);
+                hit = kick_atom(v, q.fr, q.m, D.dt, gamma, dw);
             }
             quad_book(g.w, dw, hit);
         }
         quad_gather_v(g, v);
-        r = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+        r = rattle_apply(g, q.gp, F, D.rattle_tol);
         if (k == 0) note_sweeps(S, r, FAULT_RATTLE_2, a, step, most_ra);
         float xs[4][3];
         FOR_ATOMS(j) {
@@ -635,7 +731,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half4(DevPtrs P, const float 
                 g.x[j][c] += D.dt * g.v[j][c];
             }
         }
-        r = shake(g, xs, gp, D.dt, D.shake_tol, D.constraint_max_iter);
+        r = shake(g, xs, q.gp, D.dt, D.shake_tol, D.constraint_max_iter);
         if (k == 0) note_sweeps(S, r, FAULT_SHAKE, a, step, most_sh);
         PRISM_OWN_ROW(g.x, k, x);
         PRISM_OWN_ROW(g.v, k, v);
@@ -643,22 +739,37 @@ __global__ __launch_bounds__(64 * IW) void k_first_half4(DevPtrs P, const float 
             check_atom(S, x, v, a, step);
             P.x[a] = make_float4(x[0], x[1], x[2], 0.0f);
             P.v[a] = make_float4(v[0], v[1], v[2], 0.0f);
-            float *q = (float *)&P.xq[slot];
-            q[0] = x[0]; q[1] = x[1]; q[2] = x[2];
-            const float dx = x[0] - xb.x, dy = x[1] - xb.y, dz = x[2] - xb.z;
+            float *xq = (float *)&P.xq[q.slot];
+            xq[0] = x[0]; xq[1] = x[1]; xq[2] = x[2];
+            const float dx = x[0] - q.xb[0], dy = x[1] - q.xb[1], dz = x[2] - q.xb[2];
             moved = dx * dx + dy * dy + dz * dz > D.half_skin2;
         }
         if (k == 0) P.work_g[gidx] = g.w;
     }
-    publish_sweeps(P.sweeps + 2 * gw, prev_sh, most_sh);
-    publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
-    const bool due = (long long)step - S.rebuild_step >= (long long)D.rebuild_every;
+    publish_sweeps(P.sweeps + 2 * gw, q.prev_sh, most_sh);
+    publish_sweeps(P.sweeps + 2 * gw + 1, q.prev_ra, most_ra);
+    const bool due = (long long)step - V.rebuild_step >= (long long)D.rebuild_every;
     if (__ballot(moved) != 0ull || due) {
         if (lane == 0) {
             S.rebuild_flag = 1;
             P.counters->any_rebuild = 1;
         }
     }
+}
+
+__global__ __launch_bounds__(64 * IW) void k_first_half4(DevPtrs P, const float *noise) {
+    const int lane = threadIdx.x & 63;
+    const int kw = blockIdx.x * IW + wave_id();
+    const int gw = kw >> 2;
+    if (gw >= P.n_waves_arena) return;
+    const int s = P.wave_sys[gw];
+    if (s < 0) return;
+    const SysDesc &D = P.sys[s];
+    if (!D.active) return;
+    const int gidx = gw * 64 + (kw & 3) * 16 + (lane >> 2);
+    QuadAtom q;
+    load_quad<true>(P, gw, gidx, lane & 3, q);
+    first_half4(P, noise, gw, gidx, lane, D, P.sst[s], step_view(P.sst[s]), q);
 }
 
 // ---- second half: KICK_NP1, RATTLE_BEFORE_OU_NP1, OU_HALF_NP1, the velocity
@@ -675,12 +786,17 @@ __global__ __launch_bounds__(64 * IW) void k_first_half4(DevPtrs P, const float 
 // count over the thousands of workgroups of a large arena would pay on every
 // step.  A workgroup that published adds HS_PUBLISHED, so the last of its
 // group knows to pass the writes on.
-__device__ inline void step_end(const DevPtrs &P, bool publish) {
+// The count: in every thread of the workgroup, 0 when another workgroup
+// retires last; in the last one 1, or 3 when some workgroup of the launch
+// published.  `after` is a value the caller loaded that must be in hand
+// before this workgroup is counted.
+__device__ inline int step_end_arrive(const DevPtrs &P, bool publish, unsigned after = 0u) {
     __shared__ int last;
     const int pub = __syncthreads_or(publish);
     if (pub) __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
+        asm volatile("" ::"v"(after) : "memory");
         const unsigned g = blockIdx.x / HS_GROUP;
         const unsigned gsize = min((unsigned)HS_GROUP, gridDim.x - g * HS_GROUP);
         const unsigned n_groups = (gridDim.x + HS_GROUP - 1) / HS_GROUP;
@@ -688,19 +804,24 @@ __device__ inline void step_end(const DevPtrs &P, bool publish) {
         int is_last = 0;
         if ((old & (HS_PUBLISHED - 1u)) == gsize - 1) {  // the group's last workgroup
             P.hs_group[g] = 0u;  // for the next launch; no other workgroup touches it now
+            const bool gpub = pub || old >= HS_PUBLISHED;
             if (n_groups == 1) {  // a small arena: one level
-                is_last = 1;
+                is_last = gpub ? 3 : 1;
             } else {
-                const bool gpub = pub || old >= HS_PUBLISHED;
                 if (gpub) __threadfence();
                 const unsigned o2 = atomicAdd(&P.counters->done_groups, gpub ? HS_PUBLISHED + 1u : 1u);
-                is_last = (o2 & (HS_PUBLISHED - 1u)) == n_groups - 1;
+                if ((o2 & (HS_PUBLISHED - 1u)) == n_groups - 1) is_last = (gpub || o2 >= HS_PUBLISHED) ? 3 : 1;
             }
         }
         last = is_last;
     }
     __syncthreads();
-    if (!last) return;
+    return last;
+}
+
+// The last workgroup's bookkeeping: COM sums, rebuild flags, step counters,
+// halting, the step-end stamp.
+__device__ inline void step_end_book(const DevPtrs &P) {
     __threadfence();
     for (int q = threadIdx.x; q < P.n_sys; q += blockDim.x) {
         const SysDesc &D = P.sys[q];
@@ -741,6 +862,10 @@ __device__ inline void step_end(const DevPtrs &P, bool publish) {
     }
 }
 
+__device__ inline void step_end(const DevPtrs &P, bool publish) {
+    if (step_end_arrive(P, publish)) step_end_book(P);
+}
+
 __global__ __launch_bounds__(64 * IW) void k_second_half1(DevPtrs P, const float *noise) {
     const int lane = threadIdx.x & 63;
     const int gw = blockIdx.x * IW + wave_id();
@@ -764,7 +889,9 @@ __global__ __launch_bounds__(64 * IW) void k_second_half1(DevPtrs P, const float
                 double dw;
                 if (kick_atom(g.v[k], g.f[k], g.m[k], D.dt, gamma, dw)) g.w += dw;
             }
-            int sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
+            RattleFactor F;
+            rattle_factor(g, gp, F);
+            int sw = rattle_apply(g, gp, F, D.rattle_tol);
             publish |= sw < 0;
             note_sweeps(S, sw, FAULT_RATTLE_3, g.a[0], step, most_ra);
             if (gamma != 0.0f) {
@@ -775,7 +902,7 @@ __global__ __launch_bounds__(64 * IW) void k_second_half1(DevPtrs P, const float
                 double dw;
                 if (vclamp_atom(g.v[k], g.m[k], T, dw)) g.w += dw;
             }
-            sw = rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter);
+            sw = rattle_booked(g, gp, F, D.rattle_tol);
             publish |= sw < 0;
             note_sweeps(S, sw, FAULT_RATTLE_4, g.a[0], step, most_ra);
             FOR_ATOMS(k) {
@@ -800,82 +927,262 @@ __global__ __launch_bounds__(64 * IW) void k_second_half1(DevPtrs P, const float
     step_end(P, publish);
 }
 
+// Second half of one lane's atom (the step's closing phases, see above); q.v
+// and q.w leave holding what was stored.  Returns whether the lane wrote
+// something the step end reads (COM partials, a fault).
+__device__ inline bool second_half4(const DevPtrs &P, const float *noise, int gw, int gidx, int kw, int lane,
+                                    const SysDesc &D, SysState &S, QuadAtom &q) {
+    const int k = lane & 3, a = q.a;
+    bool publish = false;
+    const unsigned long long step = S.step;
+    float T, gamma;
+    schedule(D, step + 1, T, gamma);
+    const bool next_com = D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0;
+    GroupState g;
+    g.w = q.w;
+    double mv[3] = {0.0, 0.0, 0.0};
+    int most_ra = 0;
+    if (quad_lane<0>(a) >= 0) {
+        g.n = 1 + q.gp.n_h;
+        const bool mine = k < g.n;
+        float x[3] = {q.x[0], q.x[1], q.x[2]}, v[3] = {q.v[0], q.v[1], q.v[2]};
+        float xi[3] = {0.0f, 0.0f, 0.0f};
+        if (gamma != 0.0f) ou_noise(xi, a, D, step, 2, noise);
+        quad_gather_x(g, x);
+        RattleFactor F;  // both projections below see these positions
+        rattle_factor(g, q.gp, F);
+        {
+            double dw = 0.0;
+            bool hit = false;
+            if (mine) hit = kick_atom(v, q.fr, q.m, D.dt, gamma, dw);
+            quad_book(g.w, dw, hit);
+        }
+        quad_gather_v(g, v);
+        int sw = rattle_apply(g, q.gp, F, D.rattle_tol);
+        publish |= sw < 0;
+        if (k == 0) note_sweeps(S, sw, FAULT_RATTLE_3, a, step, most_ra);
+        PRISM_OWN_ROW(g.v, k, v);
+        double d_ou = 0.0, d_vc = 0.0;
+        bool h_vc = false;
+        if (gamma != 0.0f) {
+            const float c1 = expf(-gamma * (0.5f * D.dt));
+            if (mine) d_ou = ou_apply(v, q.m, c1, T, xi);
+            quad_book(g.w, d_ou, mine);
+        }
+        if (mine) h_vc = vclamp_atom(v, q.m, T, d_vc);
+        quad_book(g.w, d_vc, h_vc);
+        quad_gather_v(g, v);
+        sw = rattle_booked(g, q.gp, F, D.rattle_tol);
+        publish |= sw < 0;
+        if (k == 0) note_sweeps(S, sw, FAULT_RATTLE_4, a, step, most_ra);
+        PRISM_OWN_ROW(g.v, k, v);
+        if (mine) {
+            if (!finite3(v)) {
+                record_fault(S, FAULT_NONFINITE, a, step);
+                publish = true;
+            }
+            P.v[a] = make_float4(v[0], v[1], v[2], 0.0f);
+        }
+        for (int c = 0; c < 3; ++c) q.v[c] = v[c];
+        g.m[0] = quad_lane<0>(q.m); g.m[1] = quad_lane<1>(q.m); g.m[2] = quad_lane<2>(q.m); g.m[3] = quad_lane<3>(q.m);
+        group_momentum(g, mv);
+        if (k == 0) P.work_g[gidx] = g.w;
+        q.w = g.w;
+    }
+    publish_sweeps(P.sweeps + 2 * gw + 1, q.prev_ra, most_ra);
+    if (next_com) {
+        for (int c = 0; c < 3; ++c) mv[c] = sum16(mv[c], 4);
+        if (lane == 0) {
+            for (int c = 0; c < 3; ++c) P.com_partial[((size_t)gw * 4 + (kw & 3)) * 3 + c] = mv[c];
+            publish = true;
+        }
+    }
+    return publish;
+}
+
 __global__ __launch_bounds__(64 * IW) void k_second_half4(DevPtrs P, const float *noise) {
-    const int lane = threadIdx.x & 63, k = lane & 3;
+    const int lane = threadIdx.x & 63;
     const int kw = blockIdx.x * IW + wave_id();
     const int gw = kw >> 2;
     bool publish = false;
     const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
     if (s >= 0 && P.sys[s].active && !P.sst[s].halted && !parked(P.sys[s], P.sst[s])) {
-        const SysDesc &D = P.sys[s];
-        SysState &S = P.sst[s];
         const int gidx = gw * 64 + (kw & 3) * 16 + (lane >> 2);
-        const int a = P.groups[gidx].atom[k];
-        const GroupParam gp = P.gparams[gidx];
-        const size_t ac = (size_t)max(a, 0);
-        const float m = P.mass[ac];
-        const float4 xa = P.x[ac], va = P.v[ac];
-        long long fr[3];
-        for (int c = 0; c < 3; ++c) fr[c] = P.f[ac * 4 + c];
-        const unsigned long long step = S.step;
-        float T, gamma;
-        schedule(D, step + 1, T, gamma);
-        const bool next_com = D.com_every > 0 && (step + 1) % (unsigned long long)D.com_every == 0;
-        const int prev_ra = P.sweeps[2 * gw + 1];
-        GroupState g;
-        g.w = P.work_g[gidx];
-        double mv[3] = {0.0, 0.0, 0.0};
-        int most_ra = 0;
-        if (quad_lane<0>(a) >= 0) {
-            g.n = 1 + gp.n_h;
-            const bool mine = k < g.n;
-            float x[3] = {xa.x, xa.y, xa.z}, v[3] = {va.x, va.y, va.z};
-            {
-                double dw = 0.0;
-                bool hit = false;
-                if (mine) hit = kick_atom(v, fr, m, D.dt, gamma, dw);
-                quad_book(g.w, dw, hit);
-            }
-            quad_gather_xv(g, x, v);
-            int sw = rattle(g, gp, D.rattle_tol, D.constraint_max_iter);
-            publish |= sw < 0;
-            if (k == 0) note_sweeps(S, sw, FAULT_RATTLE_3, a, step, most_ra);
-            PRISM_OWN_ROW(g.v, k, v);
-            double d_ou = 0.0, d_vc = 0.0;
-            bool h_vc = false;
-            if (gamma != 0.0f) {
-                const float c1 = expf(-gamma * (0.5f * D.dt));
-                if (mine) d_ou = ou_atom(v, m, a, D, c1, T, step, 2, noise);
-                quad_book(g.w, d_ou, mine);
-            }
-            if (mine) h_vc = vclamp_atom(v, m, T, d_vc);
-            quad_book(g.w, d_vc, h_vc);
-            quad_gather_v(g, v);
-            sw = rattle_booked(g, gp, D.rattle_tol, D.constraint_max_iter);
-            publish |= sw < 0;
-            if (k == 0) note_sweeps(S, sw, FAULT_RATTLE_4, a, step, most_ra);
-            PRISM_OWN_ROW(g.v, k, v);
-            if (mine) {
-                if (!finite3(v)) {
-                    record_fault(S, FAULT_NONFINITE, a, step);
-                    publish = true;
-                }
-                P.v[a] = make_float4(v[0], v[1], v[2], 0.0f);
-            }
-            g.m[0] = quad_lane<0>(m); g.m[1] = quad_lane<1>(m); g.m[2] = quad_lane<2>(m); g.m[3] = quad_lane<3>(m);
-            group_momentum(g, mv);
-            if (k == 0) P.work_g[gidx] = g.w;
-        }
-        publish_sweeps(P.sweeps + 2 * gw + 1, prev_ra, most_ra);
-        if (next_com) {
-            for (int c = 0; c < 3; ++c) mv[c] = sum16(mv[c], 4);
-            if (lane == 0) {
-                for (int c = 0; c < 3; ++c) P.com_partial[((size_t)gw * 4 + (kw & 3)) * 3 + c] = mv[c];
-                publish = true;
-            }
-        }
+        QuadAtom q;
+        load_quad<false>(P, gw, gidx, lane & 3, q);
+        publish = second_half4(P, noise, gw, gidx, kw, lane, P.sys[s], P.sst[s], q);
     }
     step_end(P, publish);
+}
+
+// ---- the step seam: the second half of step n and the first half of step
+// n + 1 in one launch (no explicit noise), joined by a grid barrier at the
+// point where the separate kernels would end and start.  The lanes keep their
+// atoms' records, positions, forces and velocities in registers across the
+// barrier: one round of loads and one launch per step fewer, bitwise the same
+// trajectory.
+//
+// The last workgroup to arrive bumps counters->seam_gen, which the others
+// wait for.  On a step on which some workgroup published (COM partials, a
+// fault latched in this launch) it first does the whole step-end bookkeeping
+// (step_end_book) and writes it back, and the bump says so (odd): the others
+// then read their system's new step state from memory at agent scope.  On the
+// other steps the new state follows from the state at kernel start
+// (view_after), so every workgroup derives it itself: the last one clears the
+// consumed rebuild flags with atomics (which complete before the bump, so no
+// first half's new flag can be overwritten), bumps, and only then writes the
+// counters from the values it read before the bump.
+//
+// The barrier needs every workgroup resident at once; the host launches the
+// seam only when the kernel's occupancy times the compute units covers the
+// grid (seam_fits).  The wait is bounded all the same: a workgroup that has
+// waited SEAM_WAIT_TICKS of the constant-rate clock latches ST_BARRIER on its
+// system and skips the first half, so the launch always ends.
+constexpr unsigned long long SEAM_WAIT_TICKS = 1ull << 26;  // ~0.67 s at 100 MHz
+
+__device__ inline unsigned long long agent_load(const void *p) {
+    return __hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A system's step-end fields at kernel start.
+struct StartState {
+    unsigned long long step;
+    long long rebuild_step;
+    int rebuild_flag, halted, status, n_rebuilds;
+};
+__device__ inline StartState start_state(const SysState &S) {
+    return StartState{S.step, S.rebuild_step, S.rebuild_flag, S.halted, S.status, S.n_rebuilds};
+}
+// What step_end_book leaves a system with when nobody published: no COM sum
+// (only COM steps publish) and no fault beyond those latched before the
+// launch.
+__device__ inline StepView view_after(const SysDesc &D, const StartState &s0) {
+    StepView V;
+    V.step = s0.step;
+    V.rebuild_step = s0.rebuild_step;
+    V.halted = s0.halted;
+    for (int c = 0; c < 3; ++c) V.com_v[c] = 0.0;  // read on COM steps only, which publish
+    if (D.active && !s0.halted && !(D.end_step != 0 && s0.step >= D.end_step)) {
+        if (s0.rebuild_flag) V.rebuild_step = (long long)s0.step;
+        V.step = s0.step + 1;
+        if (s0.status & ST_HALT) V.halted = 1;
+    }
+    return V;
+}
+
+// The last workgroup's bookkeeping on a step nobody published: the flags
+// cleared before the bump, everything else after it from the values read
+// before it.  Returns after the bump.
+__device__ inline void seam_book_quiet(const DevPtrs &P, unsigned gen0) {
+    const int q = threadIdx.x;
+    const bool mine = q < P.n_sys;
+    StartState s0{};
+    bool adv = false;
+    int cleared = 0;
+    if (mine) {
+        const SysDesc &D = P.sys[q];
+        SysState &S = P.sst[q];
+        s0 = start_state(S);
+        adv = D.active && !s0.halted && !(D.end_step != 0 && s0.step >= D.end_step);
+        if (adv && s0.rebuild_flag) cleared = 1 | atomicExch(&S.rebuild_flag, 0) | atomicExch(&S.resort_flag, 0);
+    }
+    if (threadIdx.x == 0 && P.counters->any_rebuild) cleared |= 1 | atomicExch(&P.counters->any_rebuild, 0);
+    // every clear complete (memory counter zero) before the bump
+    if (cleared) __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&P.counters->seam_gen, (gen0 & ~1u) + 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (mine && adv) {
+        SysState &S = P.sst[q];
+        if (s0.rebuild_flag) {
+            S.rebuild_step = (long long)s0.step;
+            S.n_rebuilds = s0.n_rebuilds + 1;
+        }
+        S.step = s0.step + 1;
+        if (s0.status & ST_HALT) S.halted = 1;
+    }
+    if (threadIdx.x == 0) {
+        P.counters->done_groups = 0u;
+        P.counters->any_dirty = 0;
+        const unsigned long long k = P.counters->step_ends;
+        P.step_ts[k % TS_RING] = (unsigned long long)wall_clock64();
+        P.counters->step_ends = k + 1;
+    }
+}
+
+// 0: the wait timed out; 1: the new step state follows from the state at
+// kernel start (view_after); 2: read it from memory.
+__device__ inline int seam_barrier(const DevPtrs &P, bool publish, unsigned gen0, int s) {
+    __shared__ int go;
+    const int arrived = step_end_arrive(P, publish, gen0);
+    if (arrived == 3) {
+        step_end_book(P);
+        // the bookkeeping's writers (one thread per system, thread 0 for
+        // the counters) write it back before the bump
+        if ((int)threadIdx.x < P.n_sys || threadIdx.x == 0) __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(&P.counters->seam_gen, (gen0 & ~1u) + 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            go = 2;
+        }
+    } else if (arrived == 1) {
+        seam_book_quiet(P, gen0);
+        if (threadIdx.x == 0) go = 1;
+    } else if (threadIdx.x == 0) {
+        const unsigned long long t0 = wall_clock64();
+        unsigned gen = gen0;
+        int ok = 1;
+        while ((gen = __hip_atomic_load(&P.counters->seam_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == gen0) {
+            if (wall_clock64() - t0 > SEAM_WAIT_TICKS) {
+                ok = 0;
+                atomicOr(&P.sst[max(s, 0)].status, ST_BARRIER);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        go = ok ? 1 + (int)(gen & 1u) : 0;
+    }
+    __syncthreads();
+    return go;
+}
+
+__global__ __launch_bounds__(64 * IW) void k_seam4(DevPtrs P) {
+    __shared__ StepView view;
+    const int lane = threadIdx.x & 63;
+    const int kw = blockIdx.x * IW + wave_id();
+    const int gw = kw >> 2;  // the workgroup's group wave
+    // the generation before this launch's bump: the arrival waits for it
+    // (step_end_arrive's `after`)
+    unsigned gen0 = 0u;
+    if (threadIdx.x == 0) gen0 = __hip_atomic_load(&P.counters->seam_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int s = gw < P.n_waves_arena ? P.wave_sys[gw] : -1;
+    const bool live = s >= 0 && P.sys[s].active;
+    StartState s0{};
+    if (live && threadIdx.x == 0) s0 = start_state(P.sst[s]);
+    const int gidx = gw * 64 + (kw & 3) * 16 + (lane >> 2);
+    QuadAtom q;
+    bool publish = false;
+    if (live) {
+        load_quad<true>(P, gw, gidx, lane & 3, q);
+        if (!P.sst[s].halted && !parked(P.sys[s], P.sst[s]))
+            publish = second_half4(P, nullptr, gw, gidx, kw, lane, P.sys[s], P.sst[s], q);
+    }
+    const int go = seam_barrier(P, publish, gen0, s);
+    if (!go || !live) return;
+    if (threadIdx.x == 0) {
+        if (go == 1) {
+            view = view_after(P.sys[s], s0);
+        } else {
+            const SysState &S = P.sst[s];
+            view.step = agent_load(&S.step);
+            view.rebuild_step = (long long)agent_load(&S.rebuild_step);
+            view.halted = __hip_atomic_load((int *)&S.halted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int c = 0; c < 3; ++c) view.com_v[c] = __longlong_as_double((long long)agent_load(&S.com_v[c]));
+        }
+    }
+    __syncthreads();
+    const StepView V = view;
+    first_half4(P, nullptr, gw, gidx, lane, P.sys[s], P.sst[s], V, q);
 }
 
 // Per-system reductions in FP64 (one workgroup per system, fixed order):
@@ -937,6 +1244,20 @@ void launch_second_half(const DevPtrs &P, const float *noise, hipStream_t st) {
         hipLaunchKernelGGL(k_second_half4, dim3(P.n_waves_arena), dim3(64 * IW), 0, st, P, noise);
     else
         hipLaunchKernelGGL(k_second_half1, dim3((P.n_waves_arena + IW - 1) / IW), dim3(64 * IW), 0, st, P, noise);
+}
+
+bool seam_fits(const DevPtrs &P) {
+    if (!quad_layout(P)) return false;
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seam4, 64 * IW, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return false;
+    return (long long)per_cu * cus >= (long long)P.n_waves_arena;
+}
+
+void launch_seam(const DevPtrs &P, hipStream_t st) {
+    hipLaunchKernelGGL(k_seam4, dim3(P.n_waves_arena), dim3(64 * IW), 0, st, P);
 }
 
 void launch_kinetic(const DevPtrs &P, double *out, int com_sys, hipStream_t st) {

@@ -48,8 +48,10 @@ enum : int {
     ST_MASK_OVERFLOW = 8,
     ST_EXCEPTION_COINCIDENT = 16,
     ST_DIVERGED = 32,    // a coordinate beyond MAX_COORD
+    ST_BARRIER = 64,     // the fused step seam's grid barrier timed out
     // any of these halts the system's integration at the end of the step
-    ST_HALT = ST_CONSTRAINT | ST_NONFINITE | ST_ROW_OVERFLOW | ST_MASK_OVERFLOW | ST_EXCEPTION_COINCIDENT | ST_DIVERGED,
+    ST_HALT = ST_CONSTRAINT | ST_NONFINITE | ST_ROW_OVERFLOW | ST_MASK_OVERFLOW | ST_EXCEPTION_COINCIDENT | ST_DIVERGED |
+              ST_BARRIER,
 };
 
 // Immutable per-system description (host-written).
@@ -135,7 +137,8 @@ struct EngineCounters {
     int n_units;                    // nonbonded work units claimed by the row builder
     int any_rebuild;                // some system raised its rebuild flag (cleared with any_dirty);
                                     // the rebuild kernels return at once while it is 0
-    int pad[2];
+    unsigned seam_gen;              // bumped by the step seam's grid barrier once per launch
+    int pad_c;
     unsigned long long step_ends;   // steps completed by the engine (all systems together)
 };
 constexpr int TS_RING = 4096;       // device wall-clock stamps of the most recent step ends
