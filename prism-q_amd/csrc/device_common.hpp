@@ -109,4 +109,57 @@ __device__ inline void schedule(const SysDesc &D, unsigned long long step, float
     }
 }
 
+// Value of lane (l - R) mod 16 of the same 16-lane row: DPP row_ror, no LDS
+// traffic.  Four rotations by 1, 2, 4, 8 leave every lane of a row with the
+// row's minimum or maximum.
+template <int R> __device__ inline float row_ror(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + R, 0xf, 0xf, false));
+}
+__device__ inline float row_min(float v) {
+    v = fminf(v, row_ror<1>(v));
+    v = fminf(v, row_ror<2>(v));
+    v = fminf(v, row_ror<4>(v));
+    return fminf(v, row_ror<8>(v));
+}
+__device__ inline float row_max(float v) {
+    v = fmaxf(v, row_ror<1>(v));
+    v = fmaxf(v, row_ror<2>(v));
+    v = fmaxf(v, row_ror<4>(v));
+    return fmaxf(v, row_ror<8>(v));
+}
+
+// The current bounding box of each quarter (16 sorted slots) of block b, the
+// box the nonbonded kernel culls j-atoms against, and of the whole block, the
+// box its tile skip tests: one wave, lane l on slot b*64 + l, so the four
+// 16-lane rows are the four quarters.  Padding slots take no part; a quarter
+// of padding only gets an empty box (half extent -3e38), which no j-atom
+// reaches.  Stored as (centre, half extent) pairs, the block's after its
+// quarters' (QBOX_PER_BLOCK pairs per block).
+__device__ inline void quarter_boxes(const DevPtrs &P, int b, int lane) {
+    const int slot = b * BLOCK + lane;
+    const float4 x = P.xq[slot];
+    const bool real = P.perm[slot] >= 0;
+    float lo[3] = {real ? x.x : 3.0e38f, real ? x.y : 3.0e38f, real ? x.z : 3.0e38f};
+    float hi[3] = {real ? x.x : -3.0e38f, real ? x.y : -3.0e38f, real ? x.z : -3.0e38f};
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = row_min(lo[k]);
+        hi[k] = row_max(hi[k]);
+    }
+    float4 *q = P.qbox + (size_t)b * QBOX_PER_BLOCK * 2;
+    if ((lane & (SUB - 1)) == 0) {
+        q[lane / SUB * 2] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.0f);
+        q[lane / SUB * 2 + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.0f);
+    }
+    for (int k = 0; k < 3; ++k) {  // across the four rows
+        lo[k] = fminf(lo[k], __shfl_xor(lo[k], 16));
+        lo[k] = fminf(lo[k], __shfl_xor(lo[k], 32));
+        hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], 16));
+        hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], 32));
+    }
+    if (lane == 0) {
+        q[QUARTERS * 2] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.0f);
+        q[QUARTERS * 2 + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.0f);
+    }
+}
+
 }  // namespace prism

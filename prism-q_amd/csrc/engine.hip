@@ -341,6 +341,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream, con
     P_.lj = dalloc<float2>(A);
     P_.bb_c = dalloc<float4>(B);
     P_.bb_h = dalloc<float4>(B);
+    P_.qbox = dalloc<float4>((size_t)B * QBOX_PER_BLOCK * 2);
     d_block_sys_ = dalloc<int>(B);
     P_.row_n = dalloc<int>(B);
     P_.row_j = dalloc<int>((size_t)B * MAX_ROW);
@@ -670,6 +671,9 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     D.cutoff2 = (float)(p.cutoff_A * p.cutoff_A);
     D.list_cutoff = (float)(p.cutoff_A + p.skin_A);
     D.half_skin2 = (float)(0.25 * p.skin_A * p.skin_A);
+    D.inv_rc2 = 1.0f / D.cutoff2;
+    D.inv_rc = 1.0f / std::sqrt(D.cutoff2);
+    D.reach2 = (D.cutoff + 0.001f) * (D.cutoff + 0.001f);
     D.shake_tol = p.shake_tol;
     D.rattle_tol = p.rattle_tol;
     D.inv_total_mass = 1.0 / mt;
@@ -1374,7 +1378,7 @@ int Engine::time_nonbonded(int iters, double *mean_ms) {
     if (int rc = hipcheck(hipEventCreate(&b), "event")) return rc;
     launch_nonbonded(P_, false, stream_);  // warm
     hipEventRecord(a, stream_);
-    for (int k = 0; k < iters; ++k) launch_nonbonded(P_, false, stream_);
+    for (int k = 0; k < iters; ++k) launch_nonbonded(P_, false, stream_, nullptr, nullptr, false);
     hipEventRecord(b, stream_);
     hipEventSynchronize(b);
     float ms = 0.0f;

@@ -20,6 +20,8 @@ struct DevPtrs {
     float2 *lj;                     // sorted (sigma/2, sqrt(24 eps))
     // blocks
     float4 *bb_c, *bb_h;            // build-time box centre / half extent
+    float4 *qbox;                   // [B][QBOX_PER_BLOCK][2] current boxes of each block's quarters and of
+                                    // the block: centre, half extent
     const int *block_sys;           // system of each block (-1 padding)
     int *row_n, *row_j, *row_m;     // [B], [B][MAX_ROW], [B][MAX_ROW]
     int *unit_count;                // [B] nonbonded work units of each block's row
@@ -103,17 +105,22 @@ struct RefGrid {                    // reference cell grid of the pair-list pari
 // exception (all terms, no energies) into the contribution slots, as
 // launch_bonded(P, PRISM_MD_TERM_ALL, false) does; the MD step uses it.
 void launch_sort(const DevPtrs &P, bool with_bonded, hipStream_t s);
+// launch_rows also forms the quarter boxes (P.qbox) of the step's nonbonded
+// launch; launch_quarter_boxes forms them alone.
 void launch_rows(const DevPtrs &P, hipStream_t s);
+void launch_quarter_boxes(const DevPtrs &P, hipStream_t s);
 void launch_consume(const DevPtrs &P, int step_offset, hipStream_t s);
 // slot_of / xq / lj of system s's n_slots sorted slots from its uploaded perm
 void launch_restore_slots(const DevPtrs &P, int s, int n_slots, hipStream_t st);
 // start / stop: optional HIP events recorded with the launch, bracketing the
-// kernel's execution (sampled timing)
+// kernel's execution (sampled timing).  boxes: form the quarter boxes first
+// (false only where they are current, as for a repeated timing launch).
 void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s, hipEvent_t start = nullptr,
-                      hipEvent_t stop = nullptr);
+                      hipEvent_t stop = nullptr, bool boxes = true);
 // The MD step's evaluation: nonbonded partials and the bonded contribution
 // slots (written beforehand, see launch_sort) added straight into the force
-// accumulators, which k_first_half zeroed; no gather follows.
+// accumulators, which k_first_half zeroed; no gather follows.  The quarter
+// boxes are the step's launch_rows'.
 void launch_nonbonded_direct(const DevPtrs &P, hipStream_t s, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);

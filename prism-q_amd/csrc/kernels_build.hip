@@ -389,9 +389,13 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
     __shared__ int row_lds[ROW_WAVES][MAX_ROW];
     __shared__ int flag_lds[ROW_WAVES][MAX_ROW];
     __shared__ unsigned long long mask_lds[ROW_WAVES][MAX_MASKS][64];
-    if (P.counters->any_dirty == 0 || b >= P.n_blocks_arena) return;
+    if (b >= P.n_blocks_arena) return;
     const int s = P.block_sys[b];
     if (s < 0 || !P.sys[s].active) return;
+    // every step: the quarters' current boxes for the nonbonded culls (after
+    // the sort kernel, so a re-sort's new slots are the ones boxed)
+    quarter_boxes(P, b, lane);
+    if (P.counters->any_dirty == 0) return;
     int cnt;
     if (P.sst[s].rebuild_flag) {
         const int tiles = build_row(P, b, resort_due(P.sys[s], P.sst[s]), wave, lane, row_lds, flag_lds, mask_lds);
@@ -407,6 +411,16 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
     const int per_q = cnt / QUARTERS;  // chunks per quarter
     const int ch = P.sys[s].chunk;  // the unit's first tile is stored, so decoding needs no descriptor
     for (int k = lane; k < cnt; k += 64) P.units[off + k] = (b << 11) | ((k / per_q) << 9) | ((k % per_q) * ch);
+}
+
+// The quarters' boxes alone, for the force evaluations outside the step (whose
+// positions were set or moved since the last k_rows).
+__global__ __launch_bounds__(256) void k_qbox(DevPtrs P) {
+    const int b = blockIdx.x * 4 + wave_id();
+    if (b >= P.n_blocks_arena) return;
+    const int s = P.block_sys[b];
+    if (s < 0 || !P.sys[s].active) return;
+    quarter_boxes(P, b, threadIdx.x & 63);
 }
 
 // Consume the rebuild flags after a rebuild between steps (the integrator
@@ -440,6 +454,10 @@ void launch_sort(const DevPtrs &P, bool with_bonded, hipStream_t st) {
 void launch_rows(const DevPtrs &P, hipStream_t st) {
     const int grid = (P.n_blocks_arena + ROW_WAVES - 1) / ROW_WAVES;
     hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), 0, st, P);
+}
+
+void launch_quarter_boxes(const DevPtrs &P, hipStream_t st) {
+    hipLaunchKernelGGL(k_qbox, dim3((P.n_blocks_arena + 3) / 4), dim3(256), 0, st, P);
 }
 
 void launch_restore_slots(const DevPtrs &P, int s, int n_slots, hipStream_t st) {

@@ -38,19 +38,19 @@ __device__ inline bool in_shell(float r2, unsigned lo_bits, unsigned span) {
 // j-atoms far outside the cutoff, so every lane's j index is valid.  Kept
 // pairs have r2 >= 0.01 (never denormal), rejected ones are zeroed by the
 // select, so the raw v_rsq_f32 is used without denormal scaling.
-template <bool ENERGY, bool MASKED>
+template <bool ENERGY, bool MASKED, class PAR>
 __device__ inline void tile_pairs(const float4 xi, const float2 li, int n_iter, int js, const float4 *jpos,
-                                  const float4 *jpar, unsigned long long excl, unsigned lo_bits, unsigned span,
+                                  const PAR *jpar, unsigned long long excl, unsigned lo_bits, unsigned span,
                                   float inv_rc, float inv_rc2, float &fx, float &fy, float &fz, float &e) {
 #pragma unroll 2
     for (int k = 0; k < n_iter; ++k) {
         const int jj = 4 * k + js;
         const float4 q = jpos[jj];
-        const float4 pr = jpar[jj];
+        const PAR pr = jpar[jj];
         const float dx = xi.x - q.x, dy = xi.y - q.y, dz = xi.z - q.z;
         const float r2 = pair_r2(dx, dy, dz);
         bool keep = in_shell(r2, lo_bits, span);
-        if (MASKED) keep = keep && !((excl >> __float_as_int(pr.z)) & 1ull);
+        if constexpr (MASKED) keep = keep && !((excl >> __float_as_int(pr.z)) & 1ull);
         const float inv_r = __builtin_amdgcn_rsqf(r2);
         const float inv_r2 = inv_r * inv_r;
         const float sig = li.x + pr.x;
@@ -74,25 +74,6 @@ __device__ inline void tile_pairs(const float4 xi, const float2 li, int n_iter, 
 
 constexpr int N_XCD = 8;
 
-// Value of lane (l - R) mod 16 of the same 16-lane row: DPP row_ror, no LDS
-// traffic.  Four rotations by 1, 2, 4, 8 leave every lane of a row with the
-// row's minimum or maximum.
-template <int R> __device__ inline float row_ror(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + R, 0xf, 0xf, false));
-}
-__device__ inline float row_min(float v) {
-    v = fminf(v, row_ror<1>(v));
-    v = fminf(v, row_ror<2>(v));
-    v = fminf(v, row_ror<4>(v));
-    return fminf(v, row_ror<8>(v));
-}
-__device__ inline float row_max(float v) {
-    v = fmaxf(v, row_ror<1>(v));
-    v = fmaxf(v, row_ror<2>(v));
-    v = fmaxf(v, row_ror<4>(v));
-    return fmaxf(v, row_ror<8>(v));
-}
-
 __device__ inline long long wave_xor_ll(long long v, int m) {
     const int lo = __shfl_xor((int)(v & 0xffffffffll), m), hi = __shfl_xor((int)(v >> 32), m);
     return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
@@ -102,10 +83,11 @@ __device__ inline long long wave_xor_ll(long long v, int m) {
 // q, chunk c of D.chunk tiles of b's row).  Lane l works on i-atom
 // b*64 + q*16 + (l & 15) against j-slot l >> 4: every instruction evaluates
 // 16 i-atoms x 4 j-atoms.  Per tile the 64 j-atoms are culled against the
-// quarter's current bounding box at the cutoff and the survivors compacted
-// into LDS.  The row is a full list (both directions), so each lane only
-// accumulates its own i-force; the four j-slot partials are summed at the end
-// of the unit and stored as the unit's partial (no atomics).
+// quarter's current bounding box at the cutoff (P.qbox, written before the
+// launch by quarter_boxes) and the survivors compacted into LDS.  The row is
+// a full list (both directions), so each lane only accumulates its own
+// i-force; the four j-slot partials are summed at the end of the unit and
+// stored as the unit's partial (no atomics).
 //
 // DIRECT (the MD step's evaluation): instead of storing the partial for a
 // separate gather, each unit converts it to int64 fixed point and adds it to
@@ -124,11 +106,11 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     __shared__ float4 jpos_s[NB_WAVES][64];
     __shared__ float4 jpar_s[NB_WAVES][64];
     __shared__ float4 rpos_s[NB_WAVES][RING];
-    __shared__ float4 rpar_s[NB_WAVES][RING];
+    __shared__ float2 rpar_s[NB_WAVES][RING];
     float4 *jpos = jpos_s[wave];
     float4 *jpar = jpar_s[wave];
     float4 *rpos = rpos_s[wave];
-    float4 *rpar = rpar_s[wave];
+    float2 *rpar = rpar_s[wave];
     const int n_units = P.counters->n_units;
     // Workgroups are dispatched round-robin over the XCDs (workgroup w on XCD
     // w % 8, each XCD with its own L2).  Each XCD takes one contiguous eighth
@@ -183,76 +165,51 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
             const int ti = min(t0 + (lane & (CHUNK_MAX - 1)), MAX_ROW - 1);
             const int my_jb = P.row_j[(size_t)b * MAX_ROW + ti];
             const int my_ms = P.row_m[(size_t)b * MAX_ROW + ti];
-            // and the build-time box of each tile's j-block
-            const float4 tb_c = P.bb_c[my_jb], tb_h = P.bb_h[my_jb];
+            // and the current box of each tile's j-block
+            const float4 tb_c = P.qbox[((size_t)my_jb * QBOX_PER_BLOCK + QUARTERS) * 2];
+            const float4 tb_h = P.qbox[((size_t)my_jb * QBOX_PER_BLOCK + QUARTERS) * 2 + 1];
             const int s = P.block_sys[b];
             const SysDesc &D = P.sys[s];
             const int nrow = P.row_n[b];
             const int t1 = min(nrow, t0 + D.chunk);
             const int c = t0 == 0 ? 0 : 1;  // first unit of the quarter or not
-            const float reach = D.cutoff + 0.001f;
-            const float reach2 = reach * reach;
+            const float reach2 = D.reach2;
             const unsigned lo_bits = __float_as_uint(0.01f);
             const unsigned span = __float_as_uint(D.cutoff2) - lo_bits;
-            const float inv_rc2 = 1.0f / D.cutoff2, inv_rc = rsqrtf(D.cutoff2);
+            const float inv_rc2 = D.inv_rc2, inv_rc = D.inv_rc;
 
             const int slot_i = b * BLOCK + q * SUB + il;
             const float4 xi = P.xq[slot_i];
             const float2 li = P.lj[slot_i];
             const int ai = P.perm[slot_i];
-            float lo[3], hi[3];
-            if (ai >= 0) {
-                lo[0] = hi[0] = xi.x; lo[1] = hi[1] = xi.y; lo[2] = hi[2] = xi.z;
-            } else {
-                lo[0] = lo[1] = lo[2] = 3.0e38f;
-                hi[0] = hi[1] = hi[2] = -3.0e38f;
-            }
-            // the four 16-lane rows hold the same 16 atoms: reduce within a row
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = row_min(lo[k]);
-                hi[k] = row_max(hi[k]);
-            }
-            const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
-            const float hx = 0.5f * (hi[0] - lo[0]), hy = 0.5f * (hi[1] - lo[1]), hz = 0.5f * (hi[2] - lo[2]);
+            // the quarter's current box, formed once per evaluation
+            // (quarter_boxes) rather than by each of its units
+            const float4 qc = P.qbox[((size_t)b * QBOX_PER_BLOCK + q) * 2];
+            const float4 qh = P.qbox[((size_t)b * QBOX_PER_BLOCK + q) * 2 + 1];
+            const float cx = qc.x, cy = qc.y, cz = qc.z;
+            const float hx = qh.x, hy = qh.y, hz = qh.z;
 
-            // The unit's live tiles: between rebuilds no atom is more than
-            // skin/2 from its build position (the rebuild trigger), so a tile
-            // whose j-block build box, widened by skin/2, lies beyond the
-            // cutoff of the quarter's current box holds no pair and is skipped
-            // (about a third of a row's tiles are, for a block quarter).
+            // The unit's live tiles: a tile whose j-block's current box lies
+            // beyond the cutoff of the quarter's current box holds no pair and
+            // is skipped (about half of a row's tiles are, for a block
+            // quarter: the row was built at cutoff + skin from build-time boxes).
             const int n_t = t1 - t0;
-            const float drift = sqrtf(D.half_skin2) + 0.01f;
-            const float tx = fmaxf(fabsf(cx - tb_c.x) - hx - tb_h.x - drift, 0.0f);
-            const float ty = fmaxf(fabsf(cy - tb_c.y) - hy - tb_h.y - drift, 0.0f);
-            const float tz = fmaxf(fabsf(cz - tb_c.z) - hz - tb_h.z - drift, 0.0f);
+            const float tx = fmaxf(fabsf(cx - tb_c.x) - hx - tb_h.x, 0.0f);
+            const float ty = fmaxf(fabsf(cy - tb_c.y) - hy - tb_h.y, 0.0f);
+            const float tz = fmaxf(fabsf(cz - tb_c.z) - hz - tb_h.z, 0.0f);
             unsigned long long tiles = __ballot(lane < n_t && tx * tx + ty * ty + tz * tz <= reach2);
             const bool any_tile = tiles != 0ull;
-            // then a software pipeline over the live tiles: the next tile's
-            // j-atoms load while this tile computes
-            int k = any_tile ? __ffsll((long long)tiles) - 1 : 0;
-            tiles &= tiles - 1ull;
-            int jb = __builtin_amdgcn_readlane(my_jb, k);
-            float4 xj = P.xq[jb * BLOCK + lane];
-            float2 lj = P.lj[jb * BLOCK + lane];
             float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
             // The ring holds `queued` survivors from `head` (0 or RING/2) on;
             // a full batch is always the contiguous [head, head + RING/2), and
             // appends wrap at most once (queued < RING/2 before an append of
             // at most 64).  Wave-uniform.
             int head = 0, queued = 0;
-            for (bool more = any_tile; more;) {
+            // One live tile k with its j-atoms (xj, lj) already loaded: cull
+            // the 64 j-atoms against the quarter's box, then evaluate a masked
+            // tile's survivors at once or queue an unmasked tile's in the ring.
+            auto tile = [&](const float4 xj, const float2 lj, int k) {
                 const int ms = __builtin_amdgcn_readlane(my_ms, k);
-                float4 xn = xj;
-                float2 ln = lj;
-                int kn = k;
-                more = tiles != 0ull;
-                if (more) {
-                    kn = __ffsll((long long)tiles) - 1;
-                    tiles &= tiles - 1ull;
-                    const int jn = __builtin_amdgcn_readlane(my_jb, kn);
-                    xn = P.xq[jn * BLOCK + lane];
-                    ln = P.lj[jn * BLOCK + lane];
-                }
                 const float ex = fmaxf(fabsf(xj.x - cx) - hx, 0.0f);
                 const float ey = fmaxf(fabsf(xj.y - cy) - hy, 0.0f);
                 const float ez = fmaxf(fabsf(xj.z - cz) - hz, 0.0f);
@@ -280,7 +237,7 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                     if (active) {
                         const int pos = (head + queued + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))) & (RING - 1);
                         rpos[pos] = xj;
-                        rpar[pos] = make_float4(lj.x, lj.y, 0.0f, 0.0f);
+                        rpar[pos] = lj;
                     }
                     queued += n_act;
                     if (queued >= RING / 2) {  // a full batch: 16 iterations of 16 i x 4 j
@@ -292,15 +249,44 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                         queued -= RING / 2;
                     }
                 }
-                xj = xn;
-                lj = ln;
-                k = kn;
+            };
+            // A software pipeline over the live tiles in two register sets
+            // (A, B): the next tile's j-atoms load while this one computes, and
+            // no registers rotate between tiles.  The next load is issued even
+            // after the last tile (it reloads that tile, unused), so the wait
+            // for the current tile never covers it.  The j-block index is
+            // wave-uniform: each load is a scalar base plus the lane's offset.
+            if (any_tile) {
+                const unsigned ul = (unsigned)lane;
+                int ka = __ffsll((long long)tiles) - 1, kb;
+                tiles &= tiles - 1ull;
+                const int ja = __builtin_amdgcn_readlane(my_jb, ka);
+                float4 xa = (P.xq + (size_t)ja * BLOCK)[ul], xb;
+                float2 la = (P.lj + (size_t)ja * BLOCK)[ul], lb;
+                for (;;) {
+                    bool more = tiles != 0ull;
+                    kb = more ? __ffsll((long long)tiles) - 1 : ka;
+                    tiles &= tiles - 1ull;
+                    const int jnb = __builtin_amdgcn_readlane(my_jb, kb);
+                    xb = (P.xq + (size_t)jnb * BLOCK)[ul];
+                    lb = (P.lj + (size_t)jnb * BLOCK)[ul];
+                    tile(xa, la, ka);
+                    if (!more) break;
+                    more = tiles != 0ull;
+                    ka = more ? __ffsll((long long)tiles) - 1 : kb;
+                    tiles &= tiles - 1ull;
+                    const int jna = __builtin_amdgcn_readlane(my_jb, ka);
+                    xa = (P.xq + (size_t)jna * BLOCK)[ul];
+                    la = (P.lj + (size_t)jna * BLOCK)[ul];
+                    tile(xb, lb, kb);
+                    if (!more) break;
+                }
             }
             if (queued > 0) {  // the unit's last, partial batch, padded to whole iterations
                 const int n_pad = (queued + 3) & ~3;
                 if (lane >= queued && lane < n_pad) {
                     rpos[head + lane] = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f);
-                    rpar[head + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    rpar[head + lane] = make_float2(0.0f, 0.0f);
                 }
                 wave_lds_fence();
                 tile_pairs<ENERGY, false>(xi, li, n_pad >> 2, js, rpos + head, rpar + head, 0ull, lo_bits, span, inv_rc,
@@ -555,7 +541,8 @@ static void launch_nb(const DevPtrs &P, hipStream_t st, hipEvent_t start = nullp
         hipLaunchKernelGGL((k_nonbonded<ENERGY, DIRECT, TM>), dim3(grid), dim3(64 * NB_WAVES), 0, st, P);
 }
 
-void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
+void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t st, hipEvent_t start, hipEvent_t stop, bool boxes) {
+    if (boxes) launch_quarter_boxes(P, st);
     if (energy) launch_nb<true, false, false>(P, st, start, stop);  // stores unit partials: no bonded slots read
     else launch_nb<false, false, false>(P, st, start, stop);
 }

@@ -22,6 +22,7 @@ constexpr int MAX_ROW = 512;         // j-blocks per i-block row (full list)
 constexpr int MAX_MASKS = 64;        // exclusion-mask tiles per row
 constexpr int SUB = 16;              // i-atoms per nonbonded wave (block quarter)
 constexpr int QUARTERS = BLOCK / SUB;
+constexpr int QBOX_PER_BLOCK = QUARTERS + 1;  // current boxes per block: its quarters', then its own
 constexpr int CHUNK_MIN = 8;         // tiles per nonbonded work unit: per system, in
 constexpr int CHUNK_MAX = 32;        // [CHUNK_MIN, CHUNK_MAX] (SysDesc::chunk)
 constexpr int CHUNKS_PER_ROW = MAX_ROW / CHUNK_MIN;
@@ -69,6 +70,9 @@ struct SysDesc {
     float cutoff, cutoff2, list_cutoff;
     float half_skin2;               // (skin/2)^2 displacement trigger
     int chunk;                      // row tiles per nonbonded work unit (CHUNK_MIN..CHUNK_MAX)
+    // derived on the host from the fields above, for the nonbonded kernel
+    float inv_rc, inv_rc2;          // 1 / cutoff, 1 / cutoff^2
+    float reach2;                   // (cutoff + 0.001)^2: bound of the box culls
     int pad_desc;
     double shake_tol, rattle_tol;
     double inv_total_mass;

@@ -122,6 +122,30 @@ def test_run_to_run_bitwise(pm):
         assert np.array_equal(a, b)
 
 
+def test_step_seam_equals_separate_halves(pm):
+    """A multi-step call joins each step's second half and the next step's
+    first half in one launch (the step seam); single-step calls run them as
+    separate kernels.  300 steps cross COM removals (every 100 steps, the
+    seam's publishing path), rebuilds and a re-sort: bitwise the same state,
+    work ledger and rebuild count either way."""
+    top = pm.Topology.generate(2000, seed=9)
+    out = []
+    for one_at_a_time in (False, True):
+        with pm.FusedEngine(top, pm.default_params(seed=77)) as eng:
+            eng.minimize(500, 1.0)
+            if one_at_a_time:
+                for _ in range(300):
+                    eng.step(1)
+            else:
+                eng.step(300)
+            st = eng.stats()
+            assert st["status"] == 0 and st["steps_done"] == 300, st
+            out.append((eng.get_positions(), eng.get_velocities(), eng.energies()["work"], st["pair_rebuilds"]))
+    a, b = out
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert a[2] == b[2] and a[3] == b[3], (a[2:], b[2:])
+
+
 def test_seed_changes_trajectory(pm):
     top = pm.Topology.generate(2000, seed=9)
     res = []
