@@ -21,6 +21,18 @@ __device__ inline double dot3(double3 a, double3 b) { return a.x * b.x + a.y * b
 __device__ inline double3 cross3(double3 a, double3 b) {
     return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+// 1/sqrt(x) and 1/x in FP64 from the hardware estimates (v_rsq_f64,
+// v_rcp_f64: about 2^-23 relative) and two Newton steps each (about 1e-16),
+// a fraction of the instructions of an IEEE square root and division.
+__device__ inline double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-x * y, y, 1.0);  // 1 - x y^2
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
 // Slot of a term's position q in the arena's contribution layout (TM:
 // term-major, the layout is a kernel template parameter).
 template <bool TM>
@@ -64,10 +76,11 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
         double3 r = make_double3(0, 0, 0);
         if ((mask & PRISM_MD_TERM_BOND) && B.i >= 0) {
             r = sub3(ld3d(P.x, B.j), ld3d(P.x, B.i));
-            const double len = sqrt(dot3(r, r));
-            if (len >= 1e-8) {
-                const double dr = len - (double)B.r0;
-                fm = -2.0 * (double)B.k * dr / len;
+            const double r2 = dot3(r, r);
+            if (r2 >= 1e-16) {  // |r| >= 1e-8
+                const double inv_len = rsqrt_nr(r2);
+                const double dr = r2 * inv_len - (double)B.r0;
+                fm = -2.0 * (double)B.k * dr * inv_len;
                 energy = (double)B.k * dr * dr;
             }
         }
@@ -84,15 +97,15 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
         if ((mask & PRISM_MD_TERM_ANGLE) && A.i >= 0) {
             const double3 pj = ld3d(P.x, A.j);
             const double3 a = sub3(ld3d(P.x, A.i), pj), b = sub3(ld3d(P.x, A.k), pj);
-            const double la = sqrt(dot3(a, a)), lb = sqrt(dot3(b, b));
-            if (la >= 1e-8 && lb >= 1e-8) {
-                double cs = dot3(a, b) / (la * lb);
+            const double a2 = dot3(a, a), b2 = dot3(b, b);
+            if (a2 >= 1e-16 && b2 >= 1e-16) {  // |a|, |b| >= 1e-8
+                const double ia = rsqrt_nr(a2), ib = rsqrt_nr(b2);
+                double cs = dot3(a, b) * ia * ib;
                 cs = fmax(-1.0, fmin(1.0, cs));
                 const double dth = acos(cs) - (double)A.theta0;
-                double sn = sqrt(1.0 - cs * cs);
-                if (sn < 1e-8) sn = 1e-8;
-                fm = -2.0 * (double)A.kf * dth / sn;
-                const double ia = 1.0 / la, ib = 1.0 / lb;
+                const double s2 = 1.0 - cs * cs;  // sin^2 theta, sin theta held at >= 1e-8
+                const double inv_sn = s2 < 1e-16 ? 1e8 : rsqrt_nr(s2);
+                fm = -2.0 * (double)A.kf * dth * inv_sn;
                 di = make_double3((b.x * ib - cs * a.x * ia) * ia, (b.y * ib - cs * a.y * ia) * ia, (b.z * ib - cs * a.z * ia) * ia);
                 dk = make_double3((a.x * ia - cs * b.x * ib) * ib, (a.y * ia - cs * b.y * ib) * ib, (a.z * ia - cs * b.z * ib) * ib);
                 energy = (double)A.kf * dth * dth;
@@ -114,18 +127,30 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
             const double3 c0 = cross3(v0, v1), c1 = cross3(v1, v2);
             const double n1 = dot3(c0, c0), n2 = dot3(c1, c1), nb = dot3(v1, v1);
             if (n1 >= 1e-16 && n2 >= 1e-16 && nb >= 1e-16) {
-                const double lb = sqrt(nb);
-                // phi from atan2 of |c0 x c1| and c0.c1: the reference's acos /
-                // asin branches (nhs_amber_fused.cu:562-580) in exact arithmetic
-                const double3 cc = cross3(c0, c1);
-                double phi = atan2(sqrt(dot3(cc, cc)), dot3(c0, c1));
-                if (dot3(v0, c1) < 0.0) phi = -phi;
+                const double inv_lb = rsqrt_nr(nb), lb = nb * inv_lb, inv_nb = inv_lb * inv_lb;
+                // c0 x c1 = v1 (v0 . c1), so phi = atan2(|v1| v0.c1, c0.c1): the
+                // reference's acos / asin branches (nhs_amber_fused.cu:562-580) in
+                // exact arithmetic, here as cos phi and sin phi over
+                // R = |c0| |c1| = sqrt(n1 n2)
+                const double inv_r = rsqrt_nr(n1 * n2), inv_r2 = inv_r * inv_r;
+                const double cp = dot3(c0, c1) * inv_r, sp = lb * dot3(v0, c1) * inv_r;
+                // cos(n phi), sin(n phi) by angle addition, then shifted by the
+                // phase: no transcendental on the term's chain
+                const int m = Q.n < 0 ? -Q.n : Q.n;
+                double cn = 1.0, sn = 0.0;
+                for (int t = 0; t < m; ++t) {
+                    const double c = cn * cp - sn * sp;
+                    sn = sn * cp + cn * sp;
+                    cn = c;
+                }
+                if (Q.n < 0) sn = -sn;
+                const double cd = Q.cos_phase, sd = Q.sin_phase;
+                const double sa = sn * cd - cn * sd;  // sin(n phi - phase)
+                const double ca = cn * cd + sn * sd;  // cos(n phi - phase)
                 const double nn = (double)Q.n;
-                double sa, ca;
-                sincos(nn * phi - (double)Q.phase, &sa, &ca);
                 const double dE = -(double)Q.kf * nn * sa;
-                const double ff0 = (-dE * lb) / n1, ff3 = (dE * lb) / n2;
-                const double ff1 = dot3(v0, v1) / nb, ff2 = dot3(v2, v1) / nb;
+                const double ff0 = -dE * lb * (n2 * inv_r2), ff3 = dE * lb * (n1 * inv_r2);  // / n1, / n2
+                const double ff1 = dot3(v0, v1) * inv_nb, ff2 = dot3(v2, v1) * inv_nb;
                 f1 = make_double3(ff0 * c0.x, ff0 * c0.y, ff0 * c0.z);
                 f4 = make_double3(ff3 * c1.x, ff3 * c1.y, ff3 * c1.z);
                 sv = make_double3(ff1 * f1.x - ff2 * f4.x, ff1 * f1.y - ff2 * f4.y, ff1 * f1.z - ff2 * f4.z);
@@ -150,7 +175,7 @@ __device__ inline void bonded_term(const DevPtrs &P, unsigned mask, int tid) {
             if (r2 < 1e-8) {  // coincident exception atoms (nhs_amber_fused.cu:3338-3343)
                 atomicOr(&P.sst[P.atom_sys[X.i]].status, ST_EXCEPTION_COINCIDENT);
             } else {
-                const double inv_r2 = 1.0 / r2, inv_r = sqrt(inv_r2);
+                const double inv_r = rsqrt_nr(r2), inv_r2 = inv_r * inv_r;
                 const double s2 = (double)X.sigma * (double)X.sigma * inv_r2, s6 = s2 * s2 * s2;
                 const double qq = COULOMB * (double)X.qq;
                 const double eps = X.eps;

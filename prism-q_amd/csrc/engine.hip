@@ -301,7 +301,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream, con
     h.gp.assign((size_t)W * 64, GroupParam{});
     h.bonds.assign(NB, Bond{-1, -1, 0.0f, 0.0f});
     h.angles.assign(NA, Angle{-1, -1, -1, 0.0f, 0.0f, {0, 0, 0}});
-    h.dih.assign(ND, Dihedral{-1, -1, -1, -1, 0, 0.0f, 0.0f, 0});
+    h.dih.assign(ND, Dihedral{-1, -1, -1, -1, 0, 1.0f, 0.0f, 0.0f});
     h.exc.assign(NE, Exception14{-1, -1, 0.0f, 0.0f, 0.0f, {0, 0, 0}});
     h.dst_b.assign(2 * NB, 0);
     h.dst_a.assign(3 * NA, 0);
@@ -589,9 +589,10 @@ int Engine::lower(int s, const PrismMdTopologyV1 *t, const PrismMdParamsV1 &p) {
     for (int d = 0; d < L.dih_cap; ++d)
         h.dih[L.dih_base + d] = d < t->n_dihedrals
             ? Dihedral{o + t->dihedral_atoms[4 * d], o + t->dihedral_atoms[4 * d + 1], o + t->dihedral_atoms[4 * d + 2],
-                       o + t->dihedral_atoms[4 * d + 3], t->dihedral_periodicity[d], t->dihedral_phase[d],
-                       t->dihedral_k[d], 0}
-            : Dihedral{-1, -1, -1, -1, 0, 0.0f, 0.0f, 0};
+                       o + t->dihedral_atoms[4 * d + 3], t->dihedral_periodicity[d],
+                       (float)std::cos((double)t->dihedral_phase[d]), (float)std::sin((double)t->dihedral_phase[d]),
+                       t->dihedral_k[d]}
+            : Dihedral{-1, -1, -1, -1, 0, 1.0f, 0.0f, 0.0f};
     for (int e = 0; e < L.exc_cap; ++e)
         h.exc[L.exc_base + e] = e < t->n_exceptions
             ? Exception14{o + t->exception_atoms[2 * e], o + t->exception_atoms[2 * e + 1],

@@ -160,7 +160,7 @@ struct GroupParam {
 
 struct Bond { int i, j; float r0, k; };
 struct Angle { int i, j, k; float theta0, kf; int pad[3]; };
-struct Dihedral { int i, j, k, l; int n; float phase, kf; int pad; };
+struct Dihedral { int i, j, k, l; int n; float cos_phase, sin_phase, kf; };  // phase as cos, sin (host, FP64)
 struct Exception14 { int i, j; float qq, sigma, eps; int pad[3]; };
 
 }  // namespace prism
