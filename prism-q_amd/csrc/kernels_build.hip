@@ -100,8 +100,9 @@ __global__ __launch_bounds__(256) void k_restore_slots(DevPtrs P, int s) {
 
 // Workgroups [0, n_sys): one per system, a full re-sort when the system's
 // rebuild is due and resort_due() holds.  Workgroups [n_sys, bonded_wg0): one
-// wave per block, a box refresh of blocks whose system rebuilds without
-// re-sorting.  Workgroups from bonded_wg0 on (only in the MD step's launch):
+// wave per block, every step the current boxes of the block's quarters (not
+// of a system that re-sorts: k_rows boxes those after the sort) and a box
+// refresh of blocks whose system rebuilds without re-sorting.  Workgroups from bonded_wg0 on (only in the MD step's launch):
 // the bonded terms and exceptions of the step's force evaluation, one per
 // thread -- they read the positions the sort leaves untouched and write
 // contribution slots that only the gather reads, and riding here saves the
@@ -119,15 +120,22 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int bonded_wg0
         bonded_term<false, TM>(P, PRISM_MD_TERM_ALL, ((int)blockIdx.x - bonded_wg0) * SORT_THREADS + tid);
         return;
     }
-    if (P.counters->any_rebuild == 0) return;  // nothing to rebuild this step: one load
     if ((int)blockIdx.x >= P.n_sys) {
+        // one wave per block: every step the quarters' current boxes for the
+        // nonbonded culls -- unless the block's system re-sorts this step,
+        // which moves atoms between slots, and k_rows boxes them after the
+        // sort -- then, on a rebuild without re-sort, the box refresh
         const int b = ((int)blockIdx.x - P.n_sys) * SORT_WAVES + wave;
         if (b >= P.n_blocks_arena) return;
         const int s = P.block_sys[b];
         if (s < 0) return;
         const SysDesc &D = P.sys[s];
         const SysState &S = P.sst[s];
-        if (!D.active || !S.rebuild_flag || resort_due(D, S)) return;
+        if (!D.active) return;
+        const bool rebuild = P.counters->any_rebuild != 0 && S.rebuild_flag;
+        const bool resort = rebuild && resort_due(D, S);
+        if (!resort) quarter_boxes(P, b, lane);
+        if (!rebuild || resort) return;
         refresh_block(P, b, lane);
         if (lane == 0 && b == D.block_begin) {
             atomicExch(&P.counters->any_dirty, 1);
@@ -135,6 +143,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int bonded_wg0
         }
         return;
     }
+    if (P.counters->any_rebuild == 0) return;  // nothing to re-sort this step: one load
     const int s = blockIdx.x;
     const SysDesc &D = P.sys[s];
     const SysState &S = P.sst[s];
@@ -389,15 +398,14 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
     __shared__ int row_lds[ROW_WAVES][MAX_ROW];
     __shared__ int flag_lds[ROW_WAVES][MAX_ROW];
     __shared__ unsigned long long mask_lds[ROW_WAVES][MAX_MASKS][64];
-    if (b >= P.n_blocks_arena) return;
+    if (P.counters->any_dirty == 0 || b >= P.n_blocks_arena) return;
     const int s = P.block_sys[b];
     if (s < 0 || !P.sys[s].active) return;
-    // every step: the quarters' current boxes for the nonbonded culls (after
-    // the sort kernel, so a re-sort's new slots are the ones boxed)
-    quarter_boxes(P, b, lane);
-    if (P.counters->any_dirty == 0) return;
     int cnt;
     if (P.sst[s].rebuild_flag) {
+        // the quarters' boxes of a re-sorted system, after its sort (the sort
+        // kernel's block waves box every other block)
+        if (resort_due(P.sys[s], P.sst[s])) quarter_boxes(P, b, lane);
         const int tiles = build_row(P, b, resort_due(P.sys[s], P.sst[s]), wave, lane, row_lds, flag_lds, mask_lds);
         const int ch = P.sys[s].chunk;
         cnt = QUARTERS * ((tiles + ch - 1) / ch);
