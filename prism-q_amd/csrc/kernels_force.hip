@@ -142,13 +142,25 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     // at the unit's end; rewritten as a wave-aggregated atomic it would be
     // read at once and the unit would start by waiting for it)
     unsigned *ctr = P.nb_claim + (size_t)(xcd * G + g) * NB_CLAIM_STRIDE + __builtin_amdgcn_mbcnt_lo(0u, 0u);
-    // The next unit is claimed when a unit starts; the claim and the next
-    // unit's code are read at the unit's end, before its force atomics, so
-    // no unit start waits for a claim, a code or the previous atomics.
+    // The next unit is claimed when a unit starts; the claim is read and the
+    // next unit's code loaded once the unit's tiles are known, and the next
+    // unit's row entries load before the unit's last batch, reductions and
+    // force atomics -- so a unit starts with its row in registers, and no
+    // unit start waits for a claim, a code or the previous atomics.
     unsigned k_pending = 0u;
     if (lane == 0) k_pending = atomicAdd(ctr, 1u);
     int k_cur = __builtin_amdgcn_readfirstlane((int)k_pending);
     int code_cur = k_cur < n_g ? __builtin_amdgcn_readfirstlane(P.units[u_begin + g + k_cur * G]) : 0;
+    // row entries of the unit's tiles, up to CHUNK_MAX of them (entries past
+    // the row are not used)
+    auto row_at = [&](int code, int &jb, int &ms) {
+        const int b = code >> 11, t0 = code & (MAX_ROW - 1);
+        const size_t at = (size_t)b * MAX_ROW + min(t0 + (lane & (CHUNK_MAX - 1)), MAX_ROW - 1);
+        jb = P.row_j[at];
+        ms = P.row_m[at];
+    };
+    int nx_jb = 0, nx_ms = -1;
+    if (k_cur < n_g) row_at(code_cur, nx_jb, nx_ms);
     for (;;) {
         if (k_cur >= n_g) {
             if (k_cur == n_g + waves_g - 1 && lane == 0) atomicExch(ctr, 0u);
@@ -160,12 +172,8 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
         int next_code = 0;
         {
             const int b = code >> 11, q = (code >> 9) & 3, t0 = code & (MAX_ROW - 1);
-            // tile indices of up to CHUNK_MAX tiles, loaded before the row length
-            // and the descriptor arrive (entries past the row are not used)
-            const int ti = min(t0 + (lane & (CHUNK_MAX - 1)), MAX_ROW - 1);
-            const int my_jb = P.row_j[(size_t)b * MAX_ROW + ti];
-            const int my_ms = P.row_m[(size_t)b * MAX_ROW + ti];
-            // and the current box of each tile's j-block
+            const int my_jb = nx_jb, my_ms = nx_ms;
+            // the current box of each tile's j-block
             const float4 tb_c = P.qbox[((size_t)my_jb * QBOX_PER_BLOCK + QUARTERS) * 2];
             const float4 tb_h = P.qbox[((size_t)my_jb * QBOX_PER_BLOCK + QUARTERS) * 2 + 1];
             const int s = P.block_sys[b];
@@ -198,6 +206,8 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
             const float ty = fmaxf(fabsf(cy - tb_c.y) - hy - tb_h.y, 0.0f);
             const float tz = fmaxf(fabsf(cz - tb_c.z) - hz - tb_h.z, 0.0f);
             unsigned long long tiles = __ballot(lane < n_t && tx * tx + ty * ty + tz * tz <= reach2);
+            k_cur = __builtin_amdgcn_readfirstlane((int)k_pending);
+            if (k_cur < n_g) next_code = P.units[u_begin + g + k_cur * G];
             const bool any_tile = tiles != 0ull;
             float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
             // The ring holds `queued` survivors from `head` (0 or RING/2) on;
@@ -282,6 +292,8 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                     if (!more) break;
                 }
             }
+            code_cur = __builtin_amdgcn_readfirstlane(next_code);
+            if (k_cur < n_g) row_at(code_cur, nx_jb, nx_ms);
             if (queued > 0) {  // the unit's last, partial batch, padded to whole iterations
                 const int n_pad = (queued + 3) & ~3;
                 if (lane >= queued && lane < n_pad) {
@@ -302,8 +314,6 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 e += __shfl_xor(e, 16);
                 e += __shfl_xor(e, 32);
             }
-            k_cur = __builtin_amdgcn_readfirstlane((int)k_pending);
-            if (k_cur < n_g) next_code = P.units[u_begin + g + k_cur * G];
             if (!DIRECT) {
                 if (js == 0) P.unit_f[(size_t)u * SUB + il] = make_float4(fx, fy, fz, ENERGY ? e : 0.0f);
             } else {
@@ -331,7 +341,6 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
                 }
             }
         }
-        code_cur = __builtin_amdgcn_readfirstlane(next_code);
     }
 }
 
