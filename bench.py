@@ -482,14 +482,14 @@ def main():
 
     eng.step(args.warmup)
     eng.kernel_timing(args.sample_every)
-    # Timed in chunks of at most 10,000 steps (a synchronisation each, ~0.1 ms):
-    # the synthetic chain at 4 fs can latch a SHAKE fault after 10^4 steps or
-    # more (the FP64 oracle fails on the same state, see the design notes); the
-    # leg then reports the rate over the steps completed before the faulting
-    # chunk, with the fault.
+    # Timed in chunks of at most 2,000 steps (a synchronisation each, ~0.1 ms,
+    # and a pair count outside the timed region): the synthetic chain at 4 fs
+    # can latch a SHAKE fault after 10^4 steps or more (the FP64 oracle fails
+    # on the same state, see the design notes); the leg then reports the rate
+    # over the steps completed before the faulting chunk, with the fault.
     done, wall, fault, pairs = 0, 0.0, None, None
     while done < args.steps:
-        n = min(10000, args.steps - done)
+        n = min(2000, args.steps - done)
         t0 = time.perf_counter()
         try:
             eng.step(n)  # returns after the engine's stream is synchronised
