@@ -567,7 +567,7 @@ __global__ __launch_bounds__(64 * IW) void k_first_half1(DevPtrs P, const float 
     GroupTail tail;
     bool moved = false;
     int most_sh = 0, most_ra = 0;
-    if (load_group<true>(P, gw * 64 + lane, g, gp, tail)) {
+    if (load_group<false>(P, gw * 64 + lane, g, gp, tail)) {
         if (com) {
             FOR_ATOMS(k) {
                 for (int c = 0; c < 3; ++c) g.v[k][c] -= (float)S.com_v[c];
@@ -594,6 +594,14 @@ __global__ __launch_bounds__(64 * IW) void k_first_half1(DevPtrs P, const float 
             }
         }
         note_sweeps(S, shake(g, xs, gp, D.dt, D.shake_tol, D.constraint_max_iter), FAULT_SHAKE, g.a[0], step, most_sh);
+        // each atom's sorted slot and build position, loaded only now: held
+        // through the constraint solves they would cost the kernel a wave
+        // per SIMD (188 VGPRs, two waves, instead of 166, three)
+        FOR_ATOMS(k) {
+            tail.slot[k] = P.slot_of[g.a[k]];
+            const float4 bb = P.x_build[g.a[k]];
+            tail.xb[k][0] = bb.x; tail.xb[k][1] = bb.y; tail.xb[k][2] = bb.z;
+        }
         FOR_ATOMS(k) {
             const int a = g.a[k];
             check_atom(S, g.x[k], g.v[k], a, step);
