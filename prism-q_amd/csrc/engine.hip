@@ -352,6 +352,7 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream, con
     P_.unit_count = dalloc<int>(B);
     P_.units = dalloc<int>((size_t)B * UNITS_PER_BLOCK);
     P_.unit_off = dalloc<int>(B);
+    P_.rows_total = dalloc<int>(B);  // at most one row-building workgroup per block
     P_.unit_f = dalloc<float4>((size_t)B * UNITS_PER_BLOCK * SUB);
     P_.contrib = dalloc<float>(std::max<size_t>(CS, 1) * 3);
     d_inc_off_ = dalloc<int>(A + 1);
@@ -864,25 +865,24 @@ void Engine::full_force() {
     ++force_evals_;
 }
 
-// Four kernels besides the rebuild pair: the bonded terms ride in the sort
-// kernel's launch (spare workgroups after the sort and refresh ones) and the
-// nonbonded kernel adds every contribution straight into the accumulators
-// the first half zeroed, so no bonded or gather kernel runs.
+// Four kernels: the bonded terms ride in the sort kernel's launch (spare
+// workgroups after the sort and refresh ones), the nonbonded kernel builds
+// the rows of a rebuild step ahead of its units and adds every contribution
+// straight into the accumulators the first half zeroed, so no rows, bonded or
+// gather kernel runs.
 void Engine::launch_step(const float *noise_n, const float *noise_np1, bool sample) {
     launch_first_half(P_, noise_n, stream_);
     launch_sort(P_, true, stream_);
-    launch_rows(P_, stream_);
     launch_forces(true, sample);
     launch_second_half(P_, noise_np1, stream_);
     ++force_evals_;
 }
 
-// The rest of a step whose first half has run: the rebuild pair, the forces,
-// then either the seam, which also runs the next step's first half, or the
-// second half alone.
+// The rest of a step whose first half has run: the sort kernel, the forces
+// (with the rows of a rebuild step), then either the seam, which also runs
+// the next step's first half, or the second half alone.
 void Engine::launch_tail(bool seam, bool sample) {
     launch_sort(P_, true, stream_);
-    launch_rows(P_, stream_);
     launch_forces(true, sample);
     if (seam) launch_seam(P_, stream_);
     else launch_second_half(P_, nullptr, stream_);
@@ -1035,7 +1035,7 @@ int Engine::check_status() {
         if (b & ST_EXCEPTION_COINCIDENT) return fail(PRISM_MD_E_NONFINITE, "coincident exception atoms" + where);
         if (b & ST_ROW_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "tile row overflow" + where);
         if (b & ST_MASK_OVERFLOW) return fail(PRISM_MD_E_CAPACITY, "exclusion-mask overflow" + where);
-        if (b & ST_BARRIER) return fail(PRISM_MD_E_HIP, "step seam: grid barrier timed out" + where);
+        if (b & ST_BARRIER) return fail(PRISM_MD_E_HIP, "grid barrier timed out (step seam or nonbonded rows)" + where);
         return fail(PRISM_MD_E_INVALID, "device status " + std::to_string(b) + where);
     }
     return PRISM_MD_OK;

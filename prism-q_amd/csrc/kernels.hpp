@@ -90,6 +90,7 @@ struct DevPtrs {
                                     // nonbonded kernel (zero between launches: each launch's last
                                     // claimer resets its counter)
     unsigned long long *step_ts;    // [TS_RING] device wall clock at the end of step k (slot k % TS_RING)
+    int *rows_total;                // [B] unit totals of the row-building workgroups of that launch
 };
 
 struct RefGrid {                    // reference cell grid of the pair-list parity mode
@@ -105,9 +106,10 @@ struct RefGrid {                    // reference cell grid of the pair-list pari
 // exception (all terms, no energies) into the contribution slots, as
 // launch_bonded(P, PRISM_MD_TERM_ALL, false) does; the MD step uses it.
 void launch_sort(const DevPtrs &P, bool with_bonded, hipStream_t s);
-// launch_sort and launch_rows also form the quarter boxes (P.qbox) of the
-// step's nonbonded launch (the sort kernel's block waves every step, k_rows
-// those of a system it re-sorted); launch_quarter_boxes forms them alone.
+// launch_sort forms the quarter boxes (P.qbox) of every block whose system
+// does not re-sort this step; the rows pass (launch_rows, or the step's
+// nonbonded launch) those of a system it re-sorted; launch_quarter_boxes
+// forms them alone.  launch_rows builds the rows of a rebuild between steps.
 void launch_rows(const DevPtrs &P, hipStream_t s);
 void launch_quarter_boxes(const DevPtrs &P, hipStream_t s);
 void launch_consume(const DevPtrs &P, int step_offset, hipStream_t s);
@@ -121,7 +123,9 @@ void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s, hipEvent_t s
 // The MD step's evaluation: nonbonded partials and the bonded contribution
 // slots (written beforehand, see launch_sort) added straight into the force
 // accumulators, which k_first_half zeroed; no gather follows.  The quarter
-// boxes are the step's launch_sort / launch_rows'.
+// boxes are the step's launch_sort's.  On a rebuild step the launch first
+// builds the rows and the unit list (launch_rows' work, the list in block
+// order) behind grid barriers, so the step launches no rows kernel.
 void launch_nonbonded_direct(const DevPtrs &P, hipStream_t s, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);

@@ -10,6 +10,7 @@
 // moved more than skin/2 or after rebuild_every steps.
 #include "bonded.hpp"
 #include "device_common.hpp"
+#include "rows.hpp"
 
 namespace prism {
 
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(256) void k_restore_slots(DevPtrs P, int s) {
 // Workgroups [0, n_sys): one per system, a full re-sort when the system's
 // rebuild is due and resort_due() holds.  Workgroups [n_sys, bonded_wg0): one
 // wave per block, every step the current boxes of the block's quarters (not
-// of a system that re-sorts: k_rows boxes those after the sort) and a box
+// of a system that re-sorts: the rows pass boxes those after the sort) and a box
 // refresh of blocks whose system rebuilds without re-sorting.  Workgroups from bonded_wg0 on (only in the MD step's launch):
 // the bonded terms and exceptions of the step's force evaluation, one per
 // thread -- they read the positions the sort leaves untouched and write
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int bonded_wg0
     if ((int)blockIdx.x >= P.n_sys) {
         // one wave per block: every step the quarters' current boxes for the
         // nonbonded culls -- unless the block's system re-sorts this step,
-        // which moves atoms between slots, and k_rows boxes them after the
+        // which moves atoms between slots, and the rows pass boxes them after the
         // sort -- then, on a rebuild without re-sort, the box refresh
         const int b = ((int)blockIdx.x - P.n_sys) * SORT_WAVES + wave;
         if (b >= P.n_blocks_arena) return;
@@ -274,151 +275,23 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort(DevPtrs P, int bonded_wg0
     }
 }
 
-// One wave per i-block: the row of j-blocks whose build-time boxes lie within
-// the list cutoff (full list, both directions, ascending j) and the mask slot
-// of each tile whose j-block holds a skipped pair (or is the block itself).
-// The masks themselves are rebuilt only when the atoms were re-sorted.
-__device__ int build_row(const DevPtrs &P, int b, bool resort, int wave, int lane, int (*row_lds)[MAX_ROW],
-                         int (*flag_lds)[MAX_ROW], unsigned long long (*mask_lds)[MAX_MASKS][64]) {
-    const int s = P.block_sys[b];
-    const SysDesc &D = P.sys[s];
-    SysState &S = P.sst[s];
-    const float4 ci = P.bb_c[b], hi = P.bb_h[b];
-    const float reach = D.list_cutoff + 0.01f;
-    const float reach2 = reach * reach;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int count = 0;
-    const int jend = D.block_begin + D.n_blocks;
-    for (int j0 = D.block_begin; j0 < jend; j0 += 64) {
-        const int jb = j0 + lane;
-        bool ok = false;
-        if (jb < jend) {
-            const float4 cj = P.bb_c[jb], hj = P.bb_h[jb];
-            const float dx = fmaxf(fabsf(ci.x - cj.x) - hi.x - hj.x, 0.0f);
-            const float dy = fmaxf(fabsf(ci.y - cj.y) - hi.y - hj.y, 0.0f);
-            const float dz = fmaxf(fabsf(ci.z - cj.z) - hi.z - hj.z, 0.0f);
-            ok = dx * dx + dy * dy + dz * dz <= reach2;
-        }
-        const unsigned long long m = __ballot(ok);
-        const int pos = count + __popcll(m & lt);
-        if (ok && pos < MAX_ROW) {
-            P.row_j[(size_t)b * MAX_ROW + pos] = jb;
-            row_lds[wave][pos] = jb;
-        }
-        count += __popcll(m);
-    }
-    if (count > MAX_ROW) {
-        if (lane == 0) atomicOr(&S.status, ST_ROW_OVERFLOW);
-        count = MAX_ROW;
-    }
-    if (lane == 0) {
-        P.row_n[b] = count;
-        P.unit_count[b] = QUARTERS * ((count + D.chunk - 1) / D.chunk);
-    }
-    int *pb = flag_lds[wave];  // the block's partner blocks, ascending
-    int n_pb = 0;
-    if (resort) {
-        // Partner blocks: the block itself (self pairs) and every block holding
-        // a skip partner of one of its atoms, found by repeated wave minimum;
-        // then one 64-bit mask word per (partner block, i-lane).  These change
-        // only when the atoms are re-sorted.
-        const int a = P.perm[b * BLOCK + lane];
-        const int e0 = a >= 0 ? P.skip_off[a] : 0, e1 = a >= 0 ? P.skip_off[a + 1] : 0;
-        int cur = -1;
-        for (;;) {
-            int cand = cur < b ? b : 0x7fffffff;
-            for (int e = e0; e < e1; ++e) {
-                const int q = P.slot_of[P.skip_idx[e]] / BLOCK;
-                if (q > cur && q < cand) cand = q;
-            }
-            cand = wave_min_int(cand);
-            if (cand == 0x7fffffff) break;
-            if (n_pb < MAX_MASKS && lane == 0) pb[n_pb] = cand;
-            ++n_pb;
-            cur = cand;
-        }
-        if (n_pb > MAX_MASKS) {
-            if (lane == 0) atomicOr(&S.status, ST_MASK_OVERFLOW);
-            n_pb = MAX_MASKS;
-        }
-        wave_lds_fence();
-        auto slot_in = [&](int q) -> int {  // index of block q in pb, -1 if absent
-            int lo = 0, hi2 = n_pb - 1;
-            while (lo <= hi2) {
-                const int mid = (lo + hi2) >> 1;
-                const int v = pb[mid];
-                if (v == q) return mid;
-                if (v < q) lo = mid + 1; else hi2 = mid - 1;
-            }
-            return -1;
-        };
-        // each lane owns column `lane` of the mask words: no atomics
-        for (int m = 0; m < n_pb; ++m) mask_lds[wave][m][lane] = 0ull;
-        const int mb = slot_in(b);
-        if (mb >= 0) mask_lds[wave][mb][lane] = 1ull << lane;  // self pair
-        for (int e = e0; e < e1; ++e) {
-            const int ps = P.slot_of[P.skip_idx[e]];
-            const int m = slot_in(ps / BLOCK);
-            if (m >= 0) mask_lds[wave][m][lane] |= 1ull << (ps % BLOCK);
-        }
-        unsigned long long *mrow = P.masks + (size_t)b * MAX_MASKS * 64;
-        for (int m = 0; m < n_pb; ++m) mrow[m * 64 + lane] = mask_lds[wave][m][lane];
-        if (lane < n_pb) P.pblk[(size_t)b * MAX_MASKS + lane] = pb[lane];
-        if (lane == 0) P.n_pblk[b] = n_pb;
-    } else {
-        n_pb = P.n_pblk[b];
-        if (lane < n_pb) pb[lane] = P.pblk[(size_t)b * MAX_MASKS + lane];
-    }
-    wave_lds_fence();
-    // mask slot of every tile of the row: its j-block's place among the
-    // partner blocks (both lists ascending)
-    for (int t = lane; t < count; t += 64) {
-        const int jb = row_lds[wave][t];
-        int lo = 0, hi2 = n_pb - 1, m = -1;
-        while (lo <= hi2) {
-            const int mid = (lo + hi2) >> 1;
-            const int v = pb[mid];
-            if (v == jb) { m = mid; break; }
-            if (v < jb) lo = mid + 1; else hi2 = mid - 1;
-        }
-        P.row_m[(size_t)b * MAX_ROW + t] = m;
-    }
-    return count;
-}
-
 constexpr int ROW_WAVES = 2;
-// One wave per block.  On a step where some system rebuilds: blocks of a
+// LDS that k_rows reserves and does not use: with it a compute unit holds at
+// most two of its workgroups, so the blocks claim their units roughly in
+// block order and each XCD's eighth of the unit list stays a compact region.
+// (Without it every block is in flight at once, claims come in completion
+// order, and the ensemble's nonbonded kernel was measured 13% slower.)
+constexpr size_t ROWS_LDS_RESERVE = 56 * 1024;
+// One wave per block.  On a rebuild outside the MD step (the step's
+// nonbonded launch builds its own rows, in block order): blocks of a
 // rebuilding system rebuild their row; every active block then claims a range
-// of the unit list with one atomic add and writes its units there.  The claim
-// order varies, but no result depends on it: the gather reads each block's
-// units through unit_off in chunk order.
+// of the unit list with one atomic add and writes its units there
+// (rows_block).  The claim order varies, but no result depends on it: the
+// gather reads each block's units through unit_off in chunk order.
 __global__ __launch_bounds__(64 * ROW_WAVES) void k_rows(DevPtrs P) {
-    const int wave = wave_id(), lane = threadIdx.x & 63;
-    const int b = blockIdx.x * ROW_WAVES + wave;
-    __shared__ int row_lds[ROW_WAVES][MAX_ROW];
-    __shared__ int flag_lds[ROW_WAVES][MAX_ROW];
-    __shared__ unsigned long long mask_lds[ROW_WAVES][MAX_MASKS][64];
+    const int b = blockIdx.x * ROW_WAVES + wave_id();
     if (P.counters->any_dirty == 0 || b >= P.n_blocks_arena) return;
-    const int s = P.block_sys[b];
-    if (s < 0 || !P.sys[s].active) return;
-    int cnt;
-    if (P.sst[s].rebuild_flag) {
-        // the quarters' boxes of a re-sorted system, after its sort (the sort
-        // kernel's block waves box every other block)
-        if (resort_due(P.sys[s], P.sst[s])) quarter_boxes(P, b, lane);
-        const int tiles = build_row(P, b, resort_due(P.sys[s], P.sst[s]), wave, lane, row_lds, flag_lds, mask_lds);
-        const int ch = P.sys[s].chunk;
-        cnt = QUARTERS * ((tiles + ch - 1) / ch);
-    } else {
-        cnt = P.unit_count[b];
-    }
-    int off = 0;
-    if (lane == 0) off = atomicAdd(&P.counters->n_units, cnt);
-    off = __shfl(off, 0);
-    if (lane == 0) P.unit_off[b] = off;
-    const int per_q = cnt / QUARTERS;  // chunks per quarter
-    const int ch = P.sys[s].chunk;  // the unit's first tile is stored, so decoding needs no descriptor
-    for (int k = lane; k < cnt; k += 64) P.units[off + k] = (b << 11) | ((k / per_q) << 9) | ((k % per_q) * ch);
+    rows_block(P, b, threadIdx.x & 63);
 }
 
 // The quarters' boxes alone, for the force evaluations outside the step (whose
@@ -445,6 +318,8 @@ __global__ __launch_bounds__(256) void k_consume(DevPtrs P, int step_offset) {
     if (threadIdx.x == 0) {
         P.counters->any_dirty = 0;
         P.counters->any_rebuild = 0;
+        P.counters->nb_arrive = 0u;
+        P.counters->nb_go = 0u;
     }
 }
 
@@ -461,7 +336,7 @@ void launch_sort(const DevPtrs &P, bool with_bonded, hipStream_t st) {
 
 void launch_rows(const DevPtrs &P, hipStream_t st) {
     const int grid = (P.n_blocks_arena + ROW_WAVES - 1) / ROW_WAVES;
-    hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), 0, st, P);
+    hipLaunchKernelGGL(k_rows, dim3(grid), dim3(64 * ROW_WAVES), ROWS_LDS_RESERVE, st, P);
 }
 
 void launch_quarter_boxes(const DevPtrs &P, hipStream_t st) {

@@ -15,6 +15,7 @@
 
 #include "bonded.hpp"
 #include "device_common.hpp"
+#include "rows.hpp"
 
 namespace prism {
 
@@ -79,6 +80,104 @@ __device__ inline long long wave_xor_ll(long long v, int m) {
     return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// The rows of a rebuild step inside the MD step's nonbonded launch (instead
+// of a k_rows launch on every step).  When the sort kernel marked the lists
+// dirty, the first n_builders workgroups each take K consecutive blocks, one
+// block per wave at a time: they build the rows (rows_count) and publish
+// their unit totals (rows_total); after a grid barrier each builder adds up
+// the totals before it, writes its blocks' units at those offsets -- the unit
+// list in block order, whatever the timing, so each XCD's eighth of it is a
+// compact region -- and the last builder stores n_units; after a second
+// barrier every workgroup claims units.  Only the builders count in
+// (nb_arrive, zeroed at every step end: phase p completes at p x n_builders
+// arrivals); the last of a phase stores p in nb_go, which the others poll.
+// The persistent grid is sized to the workgroups the device holds at once, so
+// every builder runs; the wait is bounded all the same (ROWS_WAIT_TICKS of the
+// constant-rate clock, then ST_BARRIER latches on every active system and
+// the launch ends without its units).  A builder's writes are released at
+// agent scope before it counts in, and every workgroup acquires after it sees
+// nb_go.  Returns false on a timeout.
+constexpr unsigned long long ROWS_WAIT_TICKS = 1ull << 26;  // ~0.67 s at 100 MHz
+
+__device__ inline bool rows_barrier(const DevPtrs &P, bool builder, unsigned n_builders, unsigned phase) {
+    __shared__ int ok;
+    __syncthreads();  // the workgroup's writes are done; one agent-scope release for all of them
+    if (threadIdx.x == 0) {
+        int good = 1;
+        bool last = false;
+        if (builder) {
+            __threadfence();
+            last = atomicAdd(&P.counters->nb_arrive, 1u) == phase * n_builders - 1u;
+            if (last) {
+                __threadfence();
+                __hip_atomic_store(&P.counters->nb_go, phase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (!last) {
+            const unsigned long long t0 = wall_clock64();
+            while (__hip_atomic_load(&P.counters->nb_go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < phase) {
+                if (wall_clock64() - t0 > ROWS_WAIT_TICKS) {
+                    good = 0;
+                    for (int q = 0; q < P.n_sys; ++q)
+                        if (P.sys[q].active) atomicOr(&P.sst[q].status, ST_BARRIER);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);  // few reads of the one word while the rows are built
+            }
+        }
+        __threadfence();
+        ok = good;
+    }
+    __syncthreads();
+    return ok;
+}
+
+// The rows phase above; false if a barrier timed out.
+__device__ inline bool rows_in_launch(const DevPtrs &P, int wave, int lane) {
+    const int B = P.n_blocks_arena;
+    if (B == 0) return true;
+    const int per = (int)gridDim.x * NB_WAVES;
+    const int K = NB_WAVES * ((B + per - 1) / per);  // blocks per builder
+    const unsigned n_builders = (unsigned)((B + K - 1) / K);
+    const bool builder = blockIdx.x < n_builders;
+    const int first = (int)blockIdx.x * K, end = min(B, first + K);
+    __shared__ int wsum[NB_WAVES];
+    if (builder) {
+        int mine = 0;
+        for (int b = first + wave; b < end; b += NB_WAVES) mine += rows_count(P, b, lane);
+        if (lane == 0) wsum[wave] = mine;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int w = 0; w < NB_WAVES; ++w) t += wsum[w];
+            P.rows_total[blockIdx.x] = t;
+        }
+    }
+    if (!rows_barrier(P, builder, n_builders, 1u)) return false;
+    if (builder) {
+        // every wave adds the totals before this builder and scans its blocks'
+        // counts itself, then writes the units of the blocks it built
+        int base = 0;
+        for (int w = lane; w < (int)blockIdx.x; w += 64) base += P.rows_total[w];
+        for (int o = 32; o > 0; o >>= 1) base += __shfl_xor(base, o);
+        int carry = base;
+        for (int c0 = first; c0 < end; c0 += 64) {
+            const int cn = c0 + lane < end ? rows_count_built(P, c0 + lane) : 0;
+            int inc = cn;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            const int excl = carry + inc - cn;
+            for (int k = wave; k < min(64, end - c0); k += NB_WAVES)
+                write_units(P, c0 + k, __shfl(excl, k), __shfl(cn, k), lane);
+            carry += __shfl(inc, 63);
+        }
+        if (blockIdx.x == n_builders - 1u && threadIdx.x == 0) P.counters->n_units = carry;
+    }
+    return rows_barrier(P, builder, n_builders, 2u);
+}
+
 // Persistent grid over the compact unit list.  A unit is (i-block b, quarter
 // q, chunk c of D.chunk tiles of b's row).  Lane l works on i-atom
 // b*64 + q*16 + (l & 15) against j-slot l >> 4: every instruction evaluates
@@ -111,6 +210,7 @@ __global__ __launch_bounds__(64 * NB_WAVES) void k_nonbonded(DevPtrs P) {
     float4 *jpar = jpar_s[wave];
     float4 *rpos = rpos_s[wave];
     float2 *rpar = rpar_s[wave];
+    if (DIRECT && P.counters->any_dirty != 0 && !rows_in_launch(P, wave, lane)) return;
     const int n_units = P.counters->n_units;
     // Workgroups are dispatched round-robin over the XCDs (workgroup w on XCD
     // w % 8, each XCD with its own L2).  Each XCD takes one contiguous eighth

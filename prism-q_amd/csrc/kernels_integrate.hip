@@ -860,6 +860,8 @@ __device__ inline void step_end_book(const DevPtrs &P) {
     if (threadIdx.x == 0) {
         P.counters->done_groups = 0u;
         P.counters->any_dirty = 0;
+        P.counters->nb_arrive = 0u;  // the nonbonded rows barrier, for the next rebuild step
+        P.counters->nb_go = 0u;
         P.counters->any_rebuild = 0;
         // the step's end on the device's constant-rate clock: the host turns
         // consecutive stamps into per-step latencies that include every gap
@@ -1112,6 +1114,8 @@ __device__ inline void seam_book_quiet(const DevPtrs &P, unsigned gen0) {
     if (threadIdx.x == 0) {
         P.counters->done_groups = 0u;
         P.counters->any_dirty = 0;
+        P.counters->nb_arrive = 0u;  // the nonbonded rows barrier, for the next rebuild step
+        P.counters->nb_go = 0u;
         const unsigned long long k = P.counters->step_ends;
         P.step_ts[k % TS_RING] = (unsigned long long)wall_clock64();
         P.counters->step_ends = k + 1;

@@ -49,7 +49,7 @@ enum : int {
     ST_MASK_OVERFLOW = 8,
     ST_EXCEPTION_COINCIDENT = 16,
     ST_DIVERGED = 32,    // a coordinate beyond MAX_COORD
-    ST_BARRIER = 64,     // the fused step seam's grid barrier timed out
+    ST_BARRIER = 64,     // a grid barrier (step seam, nonbonded rows) timed out
     // any of these halts the system's integration at the end of the step
     ST_HALT = ST_CONSTRAINT | ST_NONFINITE | ST_ROW_OVERFLOW | ST_MASK_OVERFLOW | ST_EXCEPTION_COINCIDENT | ST_DIVERGED |
               ST_BARRIER,
@@ -144,6 +144,12 @@ struct EngineCounters {
     unsigned seam_gen;              // bumped by the step seam's grid barrier once per launch
     int pad_c;
     unsigned long long step_ends;   // steps completed by the engine (all systems together)
+    // the nonbonded kernel's rows barrier (zeroed at every step end, where
+    // any_dirty is cleared), each word on a cache line of its own: the waiting workgroups
+    // poll nb_go, and neither the arrivals nor the unit claims above queue
+    // behind those reads
+    alignas(128) unsigned nb_arrive;  // row-building workgroups done
+    alignas(128) unsigned nb_go;      // set by the last of them
 };
 constexpr int TS_RING = 4096;       // device wall-clock stamps of the most recent step ends
 
