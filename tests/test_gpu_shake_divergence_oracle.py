@@ -29,7 +29,7 @@ def _start(pm, eng, top):
 
 
 @pytest.mark.gpu
-def test_nve_fault_step_reproduced_by_oracle(pm, oracle):
+def test_nve_shake_divergence_step_matches_oracle(pm, oracle):
     top, p = _nve(pm)
     fault_step = None
     with pm.FusedEngine(top, p) as eng:
