@@ -406,6 +406,10 @@ int Engine::layout(const std::vector<PrismMdSlotCapsV1> &caps, void *stream, con
     P_.n_bonds = (int)NB; P_.n_angles = (int)NA; P_.n_dih = (int)ND; P_.n_exc = (int)NE;
     P_.sys = d_desc_;
     seam_ = seam_fits(P_);
+    // a single system's step builds its rows inside the nonbonded launch (no
+    // rows launch per step); a batch keeps the rows kernel, whose re-sort
+    // steps (many systems at once) cost far less there
+    P_.rows_in_nonbonded = n_sys == 1 ? 1 : 0;
     const std::vector<int> minus1(A, -1);
     int rc = PRISM_MD_OK;
     if ((rc = upload(P_.x, h.x)) || (rc = upload(P_.v, h.v)) || (rc = upload(d_mass_, h.mass)) ||
@@ -865,24 +869,28 @@ void Engine::full_force() {
     ++force_evals_;
 }
 
-// Four kernels: the bonded terms ride in the sort kernel's launch (spare
-// workgroups after the sort and refresh ones), the nonbonded kernel builds
-// the rows of a rebuild step ahead of its units and adds every contribution
-// straight into the accumulators the first half zeroed, so no rows, bonded or
-// gather kernel runs.
+// Four kernels (five for a batch, whose rows kernel runs every step): the
+// bonded terms ride in the sort kernel's launch (spare workgroups after the
+// sort and refresh ones), a single system's nonbonded kernel builds the rows
+// of a rebuild step ahead of its units, and the nonbonded kernel adds every
+// contribution straight into the accumulators the first half zeroed, so no
+// bonded or gather kernel runs.
 void Engine::launch_step(const float *noise_n, const float *noise_np1, bool sample) {
     launch_first_half(P_, noise_n, stream_);
     launch_sort(P_, true, stream_);
+    if (!P_.rows_in_nonbonded) launch_rows(P_, stream_);
     launch_forces(true, sample);
     launch_second_half(P_, noise_np1, stream_);
     ++force_evals_;
 }
 
-// The rest of a step whose first half has run: the sort kernel, the forces
-// (with the rows of a rebuild step), then either the seam, which also runs
-// the next step's first half, or the second half alone.
+// The rest of a step whose first half has run: the sort kernel, the rows
+// (a batch's rows kernel, or inside a single system's nonbonded launch), the
+// forces, then either the seam, which also runs the next step's first half,
+// or the second half alone.
 void Engine::launch_tail(bool seam, bool sample) {
     launch_sort(P_, true, stream_);
+    if (!P_.rows_in_nonbonded) launch_rows(P_, stream_);
     launch_forces(true, sample);
     if (seam) launch_seam(P_, stream_);
     else launch_second_half(P_, nullptr, stream_);

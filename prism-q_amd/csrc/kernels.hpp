@@ -90,6 +90,7 @@ struct DevPtrs {
                                     // nonbonded kernel (zero between launches: each launch's last
                                     // claimer resets its counter)
     unsigned long long *step_ts;    // [TS_RING] device wall clock at the end of step k (slot k % TS_RING)
+    int rows_in_nonbonded;          // 1: the step's nonbonded launch builds the rows (single-system engines)
     int *rows_total;                // [B] unit totals of the row-building workgroups of that launch
 };
 
@@ -123,9 +124,10 @@ void launch_nonbonded(const DevPtrs &P, bool energy, hipStream_t s, hipEvent_t s
 // The MD step's evaluation: nonbonded partials and the bonded contribution
 // slots (written beforehand, see launch_sort) added straight into the force
 // accumulators, which k_first_half zeroed; no gather follows.  The quarter
-// boxes are the step's launch_sort's.  On a rebuild step the launch first
-// builds the rows and the unit list (launch_rows' work, the list in block
-// order) behind grid barriers, so the step launches no rows kernel.
+// boxes are the step's launch_sort's (and launch_rows').  With
+// rows_in_nonbonded the launch first builds the rows and the unit list of a
+// rebuild step (launch_rows' work, the list in block order) behind grid
+// barriers, and the step launches no rows kernel.
 void launch_nonbonded_direct(const DevPtrs &P, hipStream_t s, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_bonded(const DevPtrs &P, unsigned term_mask, bool energy, hipStream_t s);
 void launch_first_half(const DevPtrs &P, const float *noise, hipStream_t s);

@@ -282,8 +282,9 @@ constexpr int ROW_WAVES = 2;
 // (Without it every block is in flight at once, claims come in completion
 // order, and the ensemble's nonbonded kernel was measured 13% slower.)
 constexpr size_t ROWS_LDS_RESERVE = 56 * 1024;
-// One wave per block.  On a rebuild outside the MD step (the step's
-// nonbonded launch builds its own rows, in block order): blocks of a
+// One wave per block.  On a batch's MD step and on a rebuild outside the
+// step (a single system's step builds its rows inside its nonbonded launch,
+// in block order): blocks of a
 // rebuilding system rebuild their row; every active block then claims a range
 // of the unit list with one atomic add and writes its units there
 // (rows_block).  The claim order varies, but no result depends on it: the
