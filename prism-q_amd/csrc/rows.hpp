@@ -29,6 +29,19 @@ __device__ inline int pb_slot(int pb_reg, int n_pb, int q) {
     return lo < n_pb && v == q ? lo : -1;
 }
 
+// Up to SKIP_BATCH skip partners of one atom at once, e .. e + SKIP_BATCH - 1
+// below e1: their sorted slots (-1 past e1).  The index loads are issued
+// together, then the slot loads: two memory round trips per batch instead of
+// two per partner.
+constexpr int SKIP_BATCH = 16;
+__device__ inline void skip_slots(const DevPtrs &P, int e, int e1, int (&ps)[SKIP_BATCH]) {
+    int idx[SKIP_BATCH];
+#pragma unroll
+    for (int u = 0; u < SKIP_BATCH; ++u) idx[u] = e + u < e1 ? P.skip_idx[e + u] : -1;
+#pragma unroll
+    for (int u = 0; u < SKIP_BATCH; ++u) ps[u] = idx[u] >= 0 ? P.slot_of[idx[u]] : -1;
+}
+
 // One wave per i-block b: the row of j-blocks whose build-time boxes lie
 // within the list cutoff (full list, both directions, ascending j) and the
 // mask slot of each tile whose j-block holds a skipped pair (or is the block
@@ -49,9 +62,14 @@ __device__ inline int build_row(const DevPtrs &P, int b, bool resort, int lane) 
         int cur = -1;
         for (;;) {
             int cand = cur < b ? b : INT_MAX;
-            for (int e = e0; e < e1; ++e) {
-                const int q = P.slot_of[P.skip_idx[e]] / BLOCK;
-                if (q > cur && q < cand) cand = q;
+            for (int e = e0; e < e1; e += SKIP_BATCH) {
+                int ps[SKIP_BATCH];
+                skip_slots(P, e, e1, ps);
+#pragma unroll
+                for (int u = 0; u < SKIP_BATCH; ++u) {
+                    const int q = ps[u] / BLOCK;
+                    if (ps[u] >= 0 && q > cur && q < cand) cand = q;
+                }
             }
             cand = wave_min_int(cand);
             if (cand == INT_MAX) break;
@@ -75,16 +93,21 @@ __device__ inline int build_row(const DevPtrs &P, int b, bool resort, int lane) 
         const int n_e = e1 - e0, n_e_max = -wave_min_int(-n_e);
         int run_m = -1;
         unsigned long long run = 0ull;
-        for (int k = 0; k < n_e_max; ++k) {  // every lane in every round: pb_slot reads across lanes
-            const int ps = k < n_e ? P.slot_of[P.skip_idx[e0 + k]] : -BLOCK;
-            const int m = pb_slot(pb_reg, n_pb, ps / BLOCK);
-            if (k < n_e && m >= 0) {
-                if (m != run_m) {
-                    if (run_m >= 0) mrow[run_m * 64 + lane] |= run;
-                    run_m = m;
-                    run = 0ull;
+        for (int k0 = 0; k0 < n_e_max; k0 += SKIP_BATCH) {  // every lane in every round: pb_slot reads across lanes
+            int ps[SKIP_BATCH];
+            skip_slots(P, e0 + k0, e1, ps);
+#pragma unroll
+            for (int u = 0; u < SKIP_BATCH; ++u) {
+                // (rounds past n_e_max search too, with no partner: -1)
+                const int m = pb_slot(pb_reg, n_pb, ps[u] >= 0 ? ps[u] / BLOCK : -1);
+                if (ps[u] >= 0 && m >= 0) {
+                    if (m != run_m) {
+                        if (run_m >= 0) mrow[run_m * 64 + lane] |= run;
+                        run_m = m;
+                        run = 0ull;
+                    }
+                    run |= 1ull << (ps[u] % BLOCK);
                 }
-                run |= 1ull << (ps % BLOCK);
             }
         }
         if (run_m >= 0) mrow[run_m * 64 + lane] |= run;
